@@ -1,0 +1,265 @@
+#!/usr/bin/env python3
+"""Benchmark of the PV Newton-step linear solve on one MI355X.
+
+Metric: Newton refactor+solve per second, batched vs plain, on 1 GPU.
+A step is one refactor + L/U solve of every instance of the batch (one Newton
+iteration's linear algebra).  Default workload (the batched config):
+512 seeded cloud scenarios of the 40 inverters x 24 strings x 28 modules plant
+(26,920 FP64 unknowns, 80,680 nonzeros under AMD ordering, zero fill), all
+instances sharing one pattern.  The systems are the plant's companion-model
+Jacobians stamped on the GPU at the converged Newton state of each scenario
+(batched device Newton to 1e-10 V), so values are device-resident when the
+timed region starts.  Beside it on the same line:
+  * plain: the single-instance GPU path (batch 1, one kernel per level, the
+    reference's launch structure) on the same plant;
+  * dropin: the reference call sequence end to end (host assembly, copies,
+    GPU refactor/solve/update) per Newton iteration;
+  * roofline of the refactor+solve launch and the reference CPU GLU baseline.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--inverters", type=int, default=40)
+    ap.add_argument("--strings", type=int, default=24)
+    ap.add_argument("--modules", type=int, default=28)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip plain / drop-in legs")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 1 GiB device copy (counter calibration)")
+    return ap.parse_args()
+
+
+def cpu_baseline(plan_arrays, A_host, b_host, budget_s=12.0):
+    """Reference CPU GLU (GLU_CSC_CPU_sequential + Substitute_LU_Vector_b_CSC_CPU
+    compiled from the reference's own sources with its host flags), 1 thread,
+    on a bounded sample of the same instances."""
+    from oracle import oracle as O
+    kind = "reference" if O.ref(fast=True) is not None else "port"
+    s = O.CsrCsc(*plan_arrays)
+    B = A_host.shape[1]
+    # calibrate on one instance, then run a sample of ~budget_s seconds
+    t0 = time.perf_counter()
+    lu = np.ascontiguousarray(A_host[:, 0])
+    x = np.ascontiguousarray(b_host[:, 0])
+    if kind == "reference":
+        O.glu_csc_sequential(s, lu, use_ref=True, fast=True)
+        O.substitute_lu_csc(s, lu, x, use_ref=True, fast=True)
+    else:
+        O.glu_csc_sequential(s, lu, fast=True)
+        O.substitute_lu_csc(s, lu, x, fast=True)
+    one = max(time.perf_counter() - t0, 1e-6)
+    count = int(min(max(3, budget_s / one), 20 * B))
+    cols = [k % B for k in range(count)]
+    work = [(np.ascontiguousarray(A_host[:, c]), np.ascontiguousarray(b_host[:, c]))
+            for c in cols]
+    t0 = time.perf_counter()
+    for lu, x in work:
+        if kind == "reference":
+            O.glu_csc_sequential(s, lu, use_ref=True, fast=True)
+            O.substitute_lu_csc(s, lu, x, use_ref=True, fast=True)
+        else:
+            O.glu_csc_sequential(s, lu, fast=True)
+            O.substitute_lu_csc(s, lu, x, fast=True)
+    dt = time.perf_counter() - t0
+    return {"value": count / dt, "unit": "refactor+solve/s", "cores": 1, "kind": kind,
+            "sample": f"{count} instances of the batch (refactor+solve each), "
+                      f"{dt:.1f} s, 1 thread"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the path shards by instances with no "
+              "exchange inside Newton, and this build runs on a single MI355X. "
+              "Run with --gpus 1.", file=sys.stderr)
+        return 2
+    import torch
+    import phdthesis_albertojimenezruiz_amd as pv
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    B = args.batch
+    plant = pv.PV(args.inverters, args.strings, args.modules, seed=args.seed)
+    plan = plant.plan()
+    n, nnz = plant.n, plan.nnz
+    print(f"[bench] plant {args.inverters}x{args.strings}x{args.modules} n={n} nnz={nnz} "
+          f"batch={B} ncomp={plan.info('ncomp')} levels lu/l/u="
+          f"{plan.info('nlev_lu')}/{plan.info('nlev_l')}/{plan.info('nlev_u')}", flush=True)
+
+    # Converged operating point of each scenario (batched device Newton)
+    G = np.stack([plant.irradiance(s, 0.0) for s in range(B)], axis=1)
+    t0 = time.perf_counter()
+    x_n, v_n, power, nstats = plant.newton_batched(G, tol=1e-10, ladder=False)
+    t_newton = time.perf_counter() - t0
+    print(f"[bench] batched Newton: {nstats[0]:.0f} iterations, {nstats[2]:.3f} ms/iter, "
+          f"max|dV| {nstats[3]:.2e} V, total power {power.sum() / 1e6:.2f} MW "
+          f"({t_newton:.1f} s incl. setup)", flush=True)
+    sched = plan.get_schedule()
+
+    dG = torch.from_numpy(G).to(dev)
+    dV = torch.from_numpy(v_n).to(dev)
+    A = torch.empty((nnz, B), dtype=torch.float64, device=dev)
+    b = torch.empty((n, B), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream()
+    plant.stamp(B, dG, dV, A, b, -1.0, stream)
+    LU = torch.empty_like(A)
+    X = torch.empty_like(b)
+    torch.cuda.synchronize()
+
+    # ----------------------------------------------------------- batched
+    for _ in range(args.warmup):
+        plan.refactor_solve(A, LU, b, X, B, stream)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+          for _ in range(args.steps)]
+    t_all0 = torch.cuda.Event(enable_timing=True)
+    t_all1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_all0.record(stream)
+    for e0, e1 in ev:
+        e0.record(stream)
+        plan.refactor_solve(A, LU, b, X, B, stream)
+        e1.record(stream)
+    t_all1.record(stream)
+    torch.cuda.synchronize()
+    total_ms = t_all0.elapsed_time(t_all1)
+    launch_ms = [e0.elapsed_time(e1) for e0, e1 in ev]
+    avg_launch_ms = float(np.mean(launch_ms))
+    ms_per_step = total_ms / args.steps
+    value = B * args.steps / (total_ms / 1e3)
+    code = plan.status()[0]
+    if code:
+        raise RuntimeError(f"device status {code} after the timed region")
+
+    # accuracy spot check against the oracle (bit-exact expected)
+    from oracle import oracle as O
+    arrays = [plan.array(k) for k in ("csr_rowptr", "csr_colidx", "csc_colptr", "csc_rowidx",
+                                      "csr2csc", "csc2csr", "csr_diag", "csc_diag")]
+    plan_arrays = (n, *arrays)
+    s = O.CsrCsc(*plan_arrays)
+    A_host = A.cpu().numpy()
+    b_host = b.cpu().numpy()
+    X_host = X.cpu().numpy()
+    max_dx = 0.0
+    for k in (0, B // 2, B - 1):
+        lu = np.ascontiguousarray(A_host[:, k])
+        xx = np.ascontiguousarray(b_host[:, k])
+        O.glu_csc_sequential(s, lu)
+        O.substitute_lu_csc(s, lu, xx)
+        max_dx = max(max_dx, float(np.abs(xx - X_host[:, k]).max()))
+
+    bytes_launch = B * (24 * nnz + 16 * n) + (4 * nnz + 8 * n)
+    achieved = bytes_launch / (avg_launch_ms / 1e3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None, "kernel": "refactor+solve launch (schedule "
+                                            f"{sched[0]}/{sched[1]})",
+                "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_launch_ms, 5)}
+    tf = os.path.join(ROOT, "profiles", "traffic.json")
+    if os.path.exists(tf):
+        try:
+            t = json.load(open(tf))
+            if t.get("batch") == B and t.get("n") == n:
+                roofline["traffic"] = t.get("bytes_per_launch")
+        except (OSError, ValueError):
+            pass
+    print(f"[bench] batched: {ms_per_step:.4f} ms/step ({avg_launch_ms:.4f} ms/launch), "
+          f"{value:.4g} refactor+solve/s, {achieved:.0f} GB/s algorithmic, "
+          f"max|dx| vs oracle {max_dx:.1e}", flush=True)
+
+    out = {"metric": "Newton refactor+solve/sec (batched vs plain, 1 GPU) at <=1e-10 V; "
+                     "% HBM peak",
+           "value": value, "unit": "refactor+solve/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded plant + cloud scenarios, companion Jacobians at the "
+                   "converged Newton state)",
+           "config": {"workload": f"batched: {B} cloud scenarios x {args.inverters}x"
+                                  f"{args.strings}x{args.modules} plant, shared pattern",
+                      "batch": B, "n": n, "nnz_lu": nnz, "ordering": "AMD",
+                      "schedule": {"variant": sched[0], "width": sched[1]},
+                      "newton_tol_V": 1e-10, "max_abs_dx_vs_oracle": max_dx},
+           "roofline": roofline}
+
+    # ------------------------------------------------------------- plain
+    if not args.no_plain:
+        p1 = pv.PV(args.inverters, args.strings, args.modules, seed=args.seed)
+        plan1 = p1.plan()
+        plan1.set_schedule(pv.SCHED_LEVEL, 1)
+        A1 = A[:, :1].contiguous()
+        b1 = b[:, :1].contiguous()
+        LU1 = torch.empty_like(A1)
+        X1 = torch.empty_like(b1)
+        for _ in range(args.warmup):
+            plan1.refactor_solve(A1, LU1, b1, X1, 1, stream)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        reps = max(args.steps, 20)
+        e0.record(stream)
+        for _ in range(reps):
+            plan1.refactor_solve(A1, LU1, b1, X1, 1, stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        plain_ms = e0.elapsed_time(e1) / reps
+        plain = 1e3 / plain_ms
+        out["plain"] = {"value": plain, "unit": "refactor+solve/s", "ms_per_solve": plain_ms,
+                        "schedule": "one kernel per level, 1 lane per chain"}
+        out["batched_over_plain"] = value / plain
+        # drop-in: the reference call sequence per Newton iteration
+        xd, vd, st = p1.newton_dropin(G[:, 0], v=v_n[:, 0], fixed_iters=5)
+        out["dropin"] = {"ms_per_newton_iteration": st[3] / st[0],
+                         "host_assembly_ms": st[1] / st[0], "device_ms": st[2] / st[0],
+                         "note": "host assembly + H2D/D2H + GPU refactor/solve/update"}
+        print(f"[bench] plain: {plain_ms:.4f} ms/solve ({plain:.4g}/s); batched/plain "
+              f"{value / plain:.1f}x; drop-in {st[3] / st[0]:.2f} ms/iteration", flush=True)
+
+    if not args.no_cpu:
+        cb = cpu_baseline(plan_arrays, A_host, b_host)
+        try:
+            model = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+            model = [l.split(":", 1)[1].strip() for l in model.splitlines()
+                     if l.startswith("Model name")][0]
+        except (OSError, IndexError):
+            model = "unknown"
+        cb["cpu"] = model
+        out["cpu_baseline"] = cb
+        out["batched_over_cpu"] = value / cb["value"]
+        print(f"[bench] cpu baseline ({cb['kind']}): {cb['value']:.4g}/s on {model}", flush=True)
+
+    if args.copy_probe:
+        src = torch.empty(1 << 27, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out["copy_probe_bytes"] = 2 * src.numel() * 8
+
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,298 @@
+/*
+ * pvglu.h -- C-ABI of the MI355X (gfx950) PV Newton-step linear-solve path.
+ *
+ * Plain C types only (pointers, sizes, an opaque stream handle that is a
+ * hipStream_t).  Every entry point returns an int status (PVGLU_OK or a
+ * negative PVGLU_E_* code) and records a message for pvglu_last_error() on
+ * the calling thread.  Citations are to the reference entry point that each
+ * function replaces (PV/ = PVPP_Plant_Simulator/).
+ *
+ * Value layout on the device ("slot-major, instance-minor"):
+ *   values[slot * batch + instance],  slot = CSC position in the plan,
+ *   vectors[row * batch + instance],  row  = permuted unknown index.
+ * With batch == 1 this is the reference's plain CSC / dense vector layout.
+ *
+ * Numeric entry points (refactor, solve, transposed solve, Newton update,
+ * stamping) are stream-ordered: they neither synchronize nor allocate, so a
+ * caller can capture them in a hipGraph.  Numerical failures (pivot below
+ * 1e-8, damping below 1e-13) are recorded in a device status word that
+ * pvglu_status() reads (that call synchronizes the stream).
+ */
+#ifndef PVGLU_H
+#define PVGLU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  PVGLU_OK = 0,
+  PVGLU_E_SINGULAR = -1, /* |pivot| < 1e-8 (PV/GLU_CSC_CPU_levels_strings.cpp:26-30) */
+  PVGLU_E_NODIAG = -2,   /* column without a diagonal entry (:15-20) */
+  PVGLU_E_HIP = -3,      /* HIP runtime error (PV/gpu_util.h:39-47 aborts) */
+  PVGLU_E_ARG = -4,      /* invalid argument or pattern */
+  PVGLU_E_NOCONV = -5,   /* Newton damping < 1e-13 or iteration cap
+                            (PV/011_calculate_new_iteration.cpp:58-62,
+                             PV/000_PVPP_simulator.cpp:179-181) */
+  PVGLU_E_STATE = -6     /* call out of order (e.g. plan not uploaded) */
+};
+
+typedef struct pvglu_plan pvglu_plan;
+typedef struct pvglu_multi pvglu_multi;
+
+const char *pvglu_last_error(void);
+void pvglu_free(void *p);
+/* Library version and the device architecture the HIP code targets. */
+const char *pvglu_version(void);
+/* Number of visible HIP devices (0 when none; never errors). */
+int pvglu_device_count(void);
+
+/* ------------------------------------------------------------ symbolic */
+
+/* Fill-in discovery on the GPU.  Replaces obtain_fillin_matrix_CUDA
+ * (PV/005a_obtain_fillin_matrix_CUDA.cu:18-236) and its prefix sum
+ * call_cumsum_CUDA (PV/005b_cumsum_CUDA.cu:22-31).  Input: the reordered
+ * square pattern as CSR (sorted rows).  Output: the filled pattern, allocated
+ * by the library (release with pvglu_free). */
+int pvglu_obtain_fillin(int n, const int *rowptr, const int *colidx,
+                        int **filled_rowptr, int **filled_colidx,
+                        int *filled_nnz, void *stream);
+
+/* Device prefix sum (inclusive, in place) over n int32 values resident on
+ * the device.  Replaces call_cumsum_CUDA (PV/005b_cumsum_CUDA.cu:22-31).
+ * Stream-ordered; needs no scratch beyond the array itself for n <= 2^20,
+ * larger arrays use a library-held staging buffer. */
+int pvglu_inclusive_scan_device(int *d_values, int n, void *stream);
+
+/* Symbolic plan over a filled pattern: fill_fillins (PV/005c_Rellenar_Fillins.cpp:4-14),
+ * get_dependencies_GLU_3_0 / _L / _U (PV/006_get_dependencies_GLU.cpp:5-77),
+ * convert_dependence_list_in_chains_of_columns (PV/007_convert_dependencies_in_levels.cpp:146-210),
+ * convert_chains_columns_levels_into_vector (PV/008_...:4-30),
+ * get_max_numbers_per_column (PV/009_...:3-31),
+ * create_CSR_CSC_matrices (PV/010_create_CSR_CSC.cpp:15-128), i.e. the
+ * pattern half of prepare_matrices (PV/000_PVPP_simulator.cpp:33-90), plus the
+ * compiled gather schedule the kernels execute.  Host only; no device work.
+ * Fails with PVGLU_E_ARG if the pattern is not closed under elimination,
+ * lacks a diagonal, or its dependency graph has a cycle. */
+int pvglu_plan_create(int n, const int *filled_rowptr,
+                      const int *filled_colidx, pvglu_plan **out);
+void pvglu_plan_destroy(pvglu_plan *plan);
+
+/* Copies the plan's arrays to the current HIP device (the device half of
+ * struct_CPU_CUDA_arrays::create_CUDA, PV/factorize_electrical_system.h:63-98). */
+int pvglu_plan_upload(pvglu_plan *plan);
+
+/* Sizes and exported arrays, for parity tests and callers that keep the
+ * reference's struct_CPU_CUDA_arrays fields. */
+enum {
+  PVGLU_INFO_N = 0,
+  PVGLU_INFO_NNZ,
+  PVGLU_INFO_NLEV_LU,
+  PVGLU_INFO_NCHAIN_LU,
+  PVGLU_INFO_NLEV_L,
+  PVGLU_INFO_NCHAIN_L,
+  PVGLU_INFO_NLEV_U,
+  PVGLU_INFO_NCHAIN_U,
+  PVGLU_INFO_NLEV_LT,
+  PVGLU_INFO_NCHAIN_LT,
+  PVGLU_INFO_NOPS,
+  PVGLU_INFO_FUSED_LSOLVE,
+  PVGLU_INFO_NCOMP,
+  PVGLU_INFO_COUNT
+};
+int pvglu_plan_info(const pvglu_plan *plan, int what);
+
+enum {
+  PVGLU_ARR_CSR_ROWPTR = 0,     /* mat_A_CSR_start_rows                n+1 */
+  PVGLU_ARR_CSR_COLIDX,         /* mat_A_CSR_position_columns          nnz */
+  PVGLU_ARR_CSC_COLPTR,         /* mat_A_CSC_start_columns             n+1 */
+  PVGLU_ARR_CSC_ROWIDX,         /* mat_A_CSC_position_rows             nnz */
+  PVGLU_ARR_CSR2CSC,            /* mat_A_CSR_corresponding_value_in_CSC nnz */
+  PVGLU_ARR_CSC2CSR,            /* mat_A_CSC_corresponding_value_in_CSR nnz */
+  PVGLU_ARR_CSR_DIAG,           /* mat_A_CSR_start_from_diagonal       n */
+  PVGLU_ARR_CSC_DIAG,           /* mat_A_CSC_start_from_diagonal       n */
+  PVGLU_ARR_LU_START_LEVELS,    /* start_levels_unified_LU             nlev+1 */
+  PVGLU_ARR_LU_START_CHAINS,    /* start_chains_columns_unified_LU     nch+1 */
+  PVGLU_ARR_LU_CHAINS,          /* chains_columns_unified_LU           n */
+  PVGLU_ARR_L_START_LEVELS,
+  PVGLU_ARR_L_START_CHAINS,
+  PVGLU_ARR_L_CHAINS,
+  PVGLU_ARR_U_START_LEVELS,
+  PVGLU_ARR_U_START_CHAINS,
+  PVGLU_ARR_U_CHAINS,
+  PVGLU_ARR_MAX_PER_LEVEL,      /* max_elements_per_column             nlev_lu */
+  PVGLU_ARR_COUNT
+};
+/* Length of an exported array, or a negative status. */
+int pvglu_plan_array_len(const pvglu_plan *plan, int which);
+int pvglu_plan_export(const pvglu_plan *plan, int which, int *out);
+
+/* Kernel schedule: variant and instances-per-chain tile width.
+ *   PVGLU_SCHED_LEVEL      one kernel per level (reference launch structure,
+ *                          PV/factorize_electrical_system_CUDA.cu:67-144)
+ *   PVGLU_SCHED_COMPONENT  one launch for everything: one workgroup per
+ *                          (independent component, instance tile) runs all
+ *                          levels with workgroup barriers
+ * width: instances handled side by side per chain (1..64, power of two). */
+enum { PVGLU_SCHED_AUTO = 0, PVGLU_SCHED_LEVEL = 1, PVGLU_SCHED_COMPONENT = 2 };
+int pvglu_set_schedule(pvglu_plan *plan, int variant, int width);
+int pvglu_get_schedule(const pvglu_plan *plan, int *variant, int *width);
+
+/* Times the schedule candidates for this batch size on scratch buffers
+ * (allocates and synchronizes: call it outside any capture) and keeps the
+ * fastest.  Results are cached per (pattern hash, batch). */
+int pvglu_autotune(pvglu_plan *plan, int batch, int reps, double *best_ms);
+
+/* ------------------------------------------------------------- numeric */
+
+/* Level-scheduled numeric refactorization without pivoting.  Replaces
+ * factorize_electrical_system_CUDA (PV/factorize_electrical_system_CUDA.cu:61-152)
+ * and its kernel (PV/factorize_matrix_GLU_CUDA.cuh:51-105,161-211).  Reads A
+ * values, writes L\U values (unit L below the diagonal, U on and above); a may
+ * equal lu (in place, as the reference).  Per column the update order is the
+ * sequential CPU order (GLU_CSC_CPU_sequential, PV/GLU_CSC_CPU_levels_strings.cpp:108-132). */
+int pvglu_refactor(pvglu_plan *plan, const double *a, double *lu, int batch,
+                   void *stream);
+
+/* L then U level-scheduled substitution.  Replaces triangular_substitution_LU_CUDA
+ * (PV/triangular_substitution.cu:232-271).  x may equal b. */
+int pvglu_solve(pvglu_plan *plan, const double *lu, const double *b, double *x,
+                int batch, void *stream);
+
+/* Refactor followed by the solve, with the forward substitution riding in the
+ * refactor sweep when the pattern allows it. */
+int pvglu_refactor_solve(pvglu_plan *plan, const double *a, double *lu,
+                         const double *b, double *x, int batch, void *stream);
+
+/* Adjoint: solves A^T y = g with the factors of A (U^T then L^T).  No
+ * reference counterpart.  y may equal g. */
+int pvglu_solve_transpose(pvglu_plan *plan, const double *lu, const double *g,
+                          double *y, int batch, void *stream);
+
+/* Reads and clears nothing: returns the first recorded numerical failure
+ * (code PVGLU_OK if none) with its column and instance.  Synchronizes. */
+int pvglu_status(pvglu_plan *plan, int *code, int *column, int *instance);
+/* Clears the device status word (a fill kernel; capture-safe). */
+int pvglu_status_reset(pvglu_plan *plan, void *stream);
+
+/* Multi-pattern batch: groups of instances sharing one plan each (mixed plant
+ * sizes), refactored and solved in one launch from a single work list. */
+int pvglu_multi_create(int ngroups, pvglu_plan *const *plans,
+                       const int *batches, pvglu_multi **out);
+void pvglu_multi_destroy(pvglu_multi *m);
+int pvglu_multi_refactor_solve(pvglu_multi *m, const double *const *a,
+                               double *const *lu, const double *const *b,
+                               double *const *x, void *stream);
+int pvglu_multi_solve_transpose(pvglu_multi *m, const double *const *lu,
+                                const double *const *g, double *const *y,
+                                void *stream);
+
+/* -------------------------------------------------------------- Newton */
+
+/* Damped per-element voltage update of calculate_new_iteration
+ * (PV/011_calculate_new_iteration.cpp:37-86) for ne nonlinear elements in
+ * each of batch instances.  x: permuted node voltages [row][instance];
+ * pos/neg: permuted row of each element's terminals (-1 = ground);
+ * v_elem: element voltages [elem][instance], updated in place.
+ * delta_sum / delta_max: per-instance sum and max of |delta| (device,
+ * batch entries each).  Damping below 1e-13 records PVGLU_E_NOCONV in the
+ * status word of `plan`. */
+int pvglu_newton_update(pvglu_plan *plan, int ne, int batch, const int *pos,
+                        const int *neg, const double *x, double *v_elem,
+                        double *delta_sum, double *delta_max, void *stream);
+
+/* ---------------------------------------------------- PV plant harness */
+
+/* The config plants: inverters x strings x modules, single-diode modules in
+ * series, each string's combiner tied by a cable to its inverter's DC bus,
+ * each bus a Norton source (1e4 S, 600 V).  Built through the reference's
+ * circuit API (add_linear_element / add_nonlinear_voltage_source,
+ * PV/002_base_electric_system.cpp:78-108,164-173). */
+typedef struct pvglu_pv pvglu_pv;
+int pvglu_pv_create(int inverters, int strings, int modules, uint64_t seed,
+                    pvglu_pv **out);
+void pvglu_pv_destroy(pvglu_pv *pv);
+enum {
+  PVGLU_PV_N = 0,        /* unknowns (node voltages) */
+  PVGLU_PV_NMODULES,     /* nonlinear elements */
+  PVGLU_PV_NCABLES,
+  PVGLU_PV_NNZ_A,        /* nonzeros of the MNA matrix */
+  PVGLU_PV_COUNT
+};
+int pvglu_pv_info(const pvglu_pv *pv, int what);
+/* Element data: module terminals (+,-) as node indices (-1 ground), Rsh per
+ * module, cable ends and conductance, bus nodes. */
+int pvglu_pv_modules(const pvglu_pv *pv, int *pos, int *neg, double *rsh);
+int pvglu_pv_cables(const pvglu_pv *pv, int *a, int *b, double *g);
+int pvglu_pv_buses(const pvglu_pv *pv, int *bus);
+/* Module positions on the 1 km^2 grid (metres). */
+int pvglu_pv_positions(const pvglu_pv *pv, double *x, double *y);
+/* Irradiance (W/m^2) per module at time t (s) for a seeded cloud scenario
+ * (8 Gaussian shadow blobs over a 1000 W/m^2 clear sky); scenario < 0 gives
+ * the uniform 1000 W/m^2 sky. */
+int pvglu_pv_irradiance(const pvglu_pv *pv, int64_t scenario, double t,
+                        double *g_out);
+/* MNA pattern of the unpermuted system (CSR, sorted) and AMD + row-swap
+ * reordering of it: reorder_matrix (PV/004d_reorder_matrix.cpp:27-47). */
+int pvglu_pv_pattern(const pvglu_pv *pv, int *rowptr, int *colidx);
+/* The plant's symbolic plan (built on first use: reorder, GPU fill-in,
+ * plan_create, upload); owned by the plant. */
+pvglu_plan *pvglu_pv_plan(pvglu_pv *pv);
+/* Device companion-model stamping for `batch` instances: irradiance g and
+ * module voltages v ([module][instance], device) -> permuted CSC values a
+ * ([slot][instance]) and right-hand side b ([row][instance]), the
+ * create_matrix_and_vector + permutation + fill_values_CSC_only steps
+ * (PV/002b_sistema_electrico_base_matriz.cpp:141-203, PV/000_PVPP_simulator.cpp:185-210,
+ * PV/010_create_CSR_CSC.cpp:142-155) as one gather per slot.  g_virtual > 0
+ * adds the homotopy conductance across every module.  Stream-ordered. */
+int pvglu_pv_stamp(pvglu_pv *pv, int batch, const double *g, const double *v,
+                   double g_virtual, double *a, double *b, void *stream);
+/* Module current I(V) (terminal form) and its derivatives. */
+int pvglu_pv_module_current(double V, double Iph, double rsh, double *I,
+                            double *dIdV, double *dIdIph);
+
+/* Ordering: AMD 1.1 restatement on A+A^T (PV/004a_algorithm_amd_function.cpp:10-50)
+ * followed by the row-only zero-diagonal swaps (PV/004b_remove_zeros_diagonal.cpp:12-100).
+ * P[k] = original index at position k; Prow as the reference records it. */
+int pvglu_amd_order(int n, const int *rowptr, const int *colidx, int *P);
+int pvglu_reorder(int n, const int *rowptr, const int *colidx, int *P,
+                  int *Prow);
+
+/* Host drop-in Newton solve of one plant, with the reference's call sequence
+ * (PV/000_PVPP_simulator.cpp:144-235): symbolic setup once, then per
+ * iteration host assembly, permutation, CSC value fill, GPU refactor, GPU
+ * solve, un-permutation and the 011 update.  g: irradiance per module;
+ * v_module: initial/final module voltages (in/out); x_out: node voltages (n).
+ * tol: stop when max |delta| <= tol on the last homotopy stage;
+ * ladder: 1 = the reference's conductance ladder, 0 = last stage only.
+ * fixed_iters > 0 runs exactly that many iterations of the last stage.
+ * stats[0..3]: iterations, host assembly ms, device ms, total ms. */
+int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g, double *v_module,
+                           double *x_out, double tol, int ladder,
+                           int fixed_iters, double *stats);
+
+/* Batched device Newton for `batch` scenarios of one plant (shared pattern):
+ * device stamping, refactor+solve, 011 update, all stream-ordered and
+ * replayed from a hipGraph per iteration.  g: [module][instance] host array;
+ * v_module in/out [module][instance]; x_out [node][instance] (unpermuted);
+ * power_out[instance] = sum over buses of V (G_N (V - 600)).
+ * stats[0..3]: iterations, total ms, ms per iteration, max |delta|. */
+int pvglu_pv_newton_batched(pvglu_pv *pv, int batch, const double *g,
+                            double *v_module, double *x_out, double *power_out,
+                            double tol, int max_iters, int ladder,
+                            double *stats);
+
+/* Adjoint gradients of total power at a converged batched solution:
+ * dP/dG per module [module][instance] and dP/dg per cable [cable][instance].
+ * Runs the Newton solve first (same arguments as pvglu_pv_newton_batched). */
+int pvglu_pv_gradients(pvglu_pv *pv, int batch, const double *g,
+                       double *v_module, double *power_out, double *dP_dG,
+                       double *dP_dgc, double tol, int max_iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

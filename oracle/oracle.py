@@ -263,3 +263,135 @@ def module_current(V, Iph, I0, a, Rs, Rsh):
     I, dV, dI = C.c_double(), C.c_double(), C.c_double()
     lib().orc_module_current(V, Iph, I0, a, Rs, Rsh, C.byref(I), C.byref(dV), C.byref(dI))
     return I.value, dV.value, dI.value
+
+
+# ------------------------------------------------------------ PV Newton
+LADDER = (100, 10, 1, 0.9, 0.8, 0.5, 0.4, 0.3, 0.2, 0.1, 0.05, -1)  # 000:101-102
+
+
+class PVNewton:
+    """CPU restatement of the reference's Newton loop for a config plant
+    (PV/000_PVPP_simulator.cpp:163-235): companion stamping as
+    create_matrix_and_vector (PV/002b_sistema_electrico_base_matriz.cpp:141-203),
+    symmetric + row permutation, GLU_CSC_CPU_sequential, the CSC substitution,
+    un-permutation and the 011 damped update.  Modules use the implicit
+    single-diode solve (orc_module_current), not the library's Lambert W."""
+
+    RS, I0, A = 0.3, 1e-10, 1.5
+    GN, VN = 1e4, 600.0
+
+    def __init__(self, n, pos, neg, rsh, cab_a, cab_b, cab_g, buses, P, Prow):
+        self.n = n
+        self.pos, self.neg, self.rsh = pos, neg, rsh
+        self.P = np.asarray(P, _I)
+        Pinv = invert_permutation(self.P)
+        self.colmap = Pinv
+        self.rowmap = np.asarray(Prow, _I)[Pinv]
+        entries = {}
+        base = {}
+
+        def add(d, i, j, v):
+            d[(i, j)] = d.get((i, j), 0.0) + v
+
+        for a, b, g in zip(cab_a, cab_b, cab_g):
+            for (i, j, s) in ((a, a, 1), (b, b, 1), (a, b, -1), (b, a, -1)):
+                add(base, int(i), int(j), s * g)
+        self.bbase = np.zeros(n)
+        for bus in buses:
+            add(base, int(bus), int(bus), self.GN)
+            self.bbase[bus] += self.GN * self.VN
+        for e in range(len(pos)):
+            p, m = int(pos[e]), int(neg[e])
+            for (i, j) in ((p, p), (m, m), (p, m), (m, p)):
+                if i >= 0 and j >= 0:
+                    entries[(i, j)] = 1
+        for k in base:
+            entries[k] = 1
+        rows = [set() for _ in range(n)]
+        for (i, j) in entries:
+            rows[int(self.rowmap[i])].add(int(self.colmap[j]))
+        rp = [0]
+        ci = []
+        for r in rows:
+            ci.extend(sorted(r))
+            rp.append(len(ci))
+        frp, fci = symbolic_fill(n, np.array(rp, _I), np.array(ci, _I))
+        self.s = csr_csc(n, frp, fci)
+        pos_in_row = {}
+        for i in range(n):
+            for p_ in range(frp[i], frp[i + 1]):
+                pos_in_row[(i, int(fci[p_]))] = int(self.s.csr2csc[p_])
+
+        def slot(i, j):
+            return pos_in_row[(int(self.rowmap[i]), int(self.colmap[j]))]
+
+        self.abase = np.zeros(len(fci))
+        for (i, j), v in base.items():
+            self.abase[slot(i, j)] += v
+        sl, sg, el = [], [], []
+        for e in range(len(pos)):
+            p, m = int(pos[e]), int(neg[e])
+            for (i, j, s) in ((p, p, 1), (m, m, 1), (p, m, -1), (m, p, -1)):
+                if i >= 0 and j >= 0:
+                    sl.append(slot(i, j))
+                    sg.append(s)
+                    el.append(e)
+        self.st_slot = np.array(sl)
+        self.st_sign = np.array(sg, float)
+        self.st_elem = np.array(el)
+
+    def module(self, V, iph):
+        out = np.zeros((3, len(V)))
+        for e in range(len(V)):
+            out[:, e] = module_current(V[e], iph[e], self.I0, self.A, self.RS, self.rsh[e])
+        return out
+
+    def iterate(self, v, iph, gv):
+        I, dIdV, _ = self.module(v, iph)
+        f, g = -I, -dIdV
+        ieq = f - g * v
+        vals = self.abase.copy()
+        np.add.at(vals, self.st_slot, self.st_sign * g[self.st_elem])
+        if gv > 0:
+            np.add.at(vals, self.st_slot, self.st_sign * gv)
+        b = self.bbase.copy()
+        pm = self.pos >= 0
+        np.add.at(b, self.pos[pm], -ieq[pm])
+        nm = self.neg >= 0
+        np.add.at(b, self.neg[nm], ieq[nm])
+        bp = np.zeros(self.n)
+        bp[self.rowmap] = b
+        st, bad = glu_csc_sequential(self.s, vals)
+        if st != 0:
+            raise RuntimeError(f"oracle GLU status {st} at column {bad}")
+        substitute_lu_csc(self.s, vals, bp)
+        x = bp[self.colmap]
+        vn = np.where(self.pos >= 0, x[np.maximum(self.pos, 0)], 0.0) - \
+            np.where(self.neg >= 0, x[np.maximum(self.neg, 0)], 0.0)
+        st, v2, diff, mx = newton_update(v, vn)
+        if st != 0:
+            raise RuntimeError("oracle Newton: damping below 1e-13")
+        return x, v2, diff, mx, vals
+
+    def solve(self, iph, v0, ladder=True, tol=1e-10, max_iters=5000):
+        v = np.asarray(v0, float).copy()
+        it = 0
+        stages = LADDER if ladder else LADDER[-1:]
+        x = None
+        for k, gv in enumerate(stages):
+            last = k == len(stages) - 1
+            while True:
+                it += 1
+                if it > max_iters:
+                    raise RuntimeError("oracle Newton did not converge")
+                x, v, diff, mx, lu = self.iterate(v, iph, gv)
+                if not last and diff < 1e-6 * self.n:
+                    break
+                if last and mx <= tol:
+                    break
+        self.lu = lu
+        return x, v, it
+
+    def power(self, x, buses):
+        V = x[np.asarray(buses)]
+        return float(np.sum(V * self.GN * (V - self.VN)))

@@ -1,0 +1,404 @@
+"""MI355X-native PV Newton-step linear-solve path (GLU refactor + solve).
+
+Python bindings over the C-ABI of ``libpvglu.so`` (declared in
+``include/pvglu.h``).  The library is the product: plan creation, the HIP
+kernels and the PV plant harness are native code.  This module only loads it
+and converts arguments; it never computes anything itself and raises if the
+native library is missing.
+
+Device buffers are passed as raw pointers (``torch.Tensor.data_ptr()``) and
+streams as ``torch.cuda.Stream.cuda_stream``; torch is used for device memory
+and streams only.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["lib", "Plan", "PV", "PvgluError", "LIB_PATH", "reorder", "amd_order",
+           "obtain_fillin", "module_current"]
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpvglu.so")
+
+OK, E_SINGULAR, E_NODIAG, E_HIP, E_ARG, E_NOCONV, E_STATE = 0, -1, -2, -3, -4, -5, -6
+SCHED_AUTO, SCHED_LEVEL, SCHED_COMPONENT = 0, 1, 2
+
+INFO = dict(n=0, nnz=1, nlev_lu=2, nchain_lu=3, nlev_l=4, nchain_l=5, nlev_u=6,
+            nchain_u=7, nlev_lt=8, nchain_lt=9, nops=10, fused_lsolve=11, ncomp=12)
+ARRAYS = dict(csr_rowptr=0, csr_colidx=1, csc_colptr=2, csc_rowidx=3, csr2csc=4,
+              csc2csr=5, csr_diag=6, csc_diag=7, lu_start_levels=8, lu_start_chains=9,
+              lu_chains=10, l_start_levels=11, l_start_chains=12, l_chains=13,
+              u_start_levels=14, u_start_chains=15, u_chains=16, max_per_level=17)
+
+_P_INT = C.POINTER(C.c_int)
+_P_DBL = C.POINTER(C.c_double)
+_VP = C.c_void_p
+
+
+class PvgluError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"pvglu error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Loads libpvglu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with "
+                           "python -m phdthesis_albertojimenezruiz_amd.build")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "pvglu_last_error": (C.c_char_p, []),
+        "pvglu_version": (C.c_char_p, []),
+        "pvglu_free": (None, [_VP]),
+        "pvglu_device_count": (C.c_int, []),
+        "pvglu_obtain_fillin": (C.c_int, [C.c_int, _P_INT, _P_INT, C.POINTER(_P_INT),
+                                          C.POINTER(_P_INT), _P_INT, _VP]),
+        "pvglu_inclusive_scan_device": (C.c_int, [_VP, C.c_int, _VP]),
+        "pvglu_plan_create": (C.c_int, [C.c_int, _P_INT, _P_INT, C.POINTER(_VP)]),
+        "pvglu_plan_destroy": (None, [_VP]),
+        "pvglu_plan_upload": (C.c_int, [_VP]),
+        "pvglu_plan_info": (C.c_int, [_VP, C.c_int]),
+        "pvglu_plan_array_len": (C.c_int, [_VP, C.c_int]),
+        "pvglu_plan_export": (C.c_int, [_VP, C.c_int, _P_INT]),
+        "pvglu_set_schedule": (C.c_int, [_VP, C.c_int, C.c_int]),
+        "pvglu_get_schedule": (C.c_int, [_VP, _P_INT, _P_INT]),
+        "pvglu_autotune": (C.c_int, [_VP, C.c_int, C.c_int, _P_DBL]),
+        "pvglu_refactor": (C.c_int, [_VP, _VP, _VP, C.c_int, _VP]),
+        "pvglu_solve": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, _VP]),
+        "pvglu_refactor_solve": (C.c_int, [_VP, _VP, _VP, _VP, _VP, C.c_int, _VP]),
+        "pvglu_solve_transpose": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, _VP]),
+        "pvglu_status": (C.c_int, [_VP, _P_INT, _P_INT, _P_INT]),
+        "pvglu_status_reset": (C.c_int, [_VP, _VP]),
+        "pvglu_newton_update": (C.c_int, [_VP, C.c_int, C.c_int, _VP, _VP, _VP, _VP, _VP,
+                                          _VP, _VP]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(code):
+    if code < 0:
+        msg = lib().pvglu_last_error()
+        raise PvgluError(code, msg.decode() if msg else "")
+    return code
+
+
+def _ip(a):
+    return a.ctypes.data_as(_P_INT)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, np.int32)
+
+
+def _stream(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return stream
+    return getattr(stream, "cuda_stream", stream)
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (or an int address)."""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return t
+    return t.data_ptr()
+
+
+class Plan:
+    """Symbolic plan of one filled pattern (``pvglu_plan_create``)."""
+
+    def __init__(self, n, filled_rowptr, filled_colidx):
+        rp, ci = _i32(filled_rowptr), _i32(filled_colidx)
+        h = _VP()
+        _check(lib().pvglu_plan_create(int(n), _ip(rp), _ip(ci), C.byref(h)))
+        self._h = h
+        self.n = int(n)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().pvglu_plan_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    @property
+    def handle(self):
+        return self._h if self._h is not None else self._view
+
+    def info(self, what):
+        return _check(lib().pvglu_plan_info(self.handle, INFO[what]))
+
+    @property
+    def nnz(self):
+        return self.info("nnz")
+
+    def array(self, which):
+        k = ARRAYS[which]
+        m = _check(lib().pvglu_plan_array_len(self.handle, k))
+        out = np.zeros(max(m, 1), np.int32)
+        _check(lib().pvglu_plan_export(self.handle, k, _ip(out)))
+        return out[:m]
+
+    def upload(self):
+        _check(lib().pvglu_plan_upload(self.handle))
+
+    def set_schedule(self, variant, width=0):
+        _check(lib().pvglu_set_schedule(self.handle, int(variant), int(width)))
+
+    def get_schedule(self):
+        v, w = C.c_int(), C.c_int()
+        _check(lib().pvglu_get_schedule(self.handle, C.byref(v), C.byref(w)))
+        return v.value, w.value
+
+    def autotune(self, batch, reps=5):
+        best = C.c_double()
+        _check(lib().pvglu_autotune(self.handle, int(batch), int(reps), C.byref(best)))
+        return best.value
+
+    def refactor(self, a, lu, batch=1, stream=None):
+        _check(lib().pvglu_refactor(self.handle, _ptr(a), _ptr(lu), int(batch), _stream(stream)))
+
+    def solve(self, lu, b, x, batch=1, stream=None):
+        _check(lib().pvglu_solve(self.handle, _ptr(lu), _ptr(b), _ptr(x), int(batch),
+                                 _stream(stream)))
+
+    def refactor_solve(self, a, lu, b, x, batch=1, stream=None):
+        _check(lib().pvglu_refactor_solve(self.handle, _ptr(a), _ptr(lu), _ptr(b), _ptr(x),
+                                          int(batch), _stream(stream)))
+
+    def solve_transpose(self, lu, g, y, batch=1, stream=None):
+        _check(lib().pvglu_solve_transpose(self.handle, _ptr(lu), _ptr(g), _ptr(y), int(batch),
+                                           _stream(stream)))
+
+    def status(self):
+        c, col, inst = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().pvglu_status(self.handle, C.byref(c), C.byref(col), C.byref(inst)))
+        return c.value, col.value, inst.value
+
+    def status_reset(self, stream=None):
+        _check(lib().pvglu_status_reset(self.handle, _stream(stream)))
+
+    def newton_update(self, ne, batch, pos, neg, x, v, dsum, dmax, stream=None):
+        _check(lib().pvglu_newton_update(self.handle, int(ne), int(batch), _ptr(pos), _ptr(neg),
+                                         _ptr(x), _ptr(v), _ptr(dsum), _ptr(dmax),
+                                         _stream(stream)))
+
+
+def obtain_fillin(n, rowptr, colidx, stream=None):
+    """GPU fill-in discovery (``pvglu_obtain_fillin``): filled CSR pattern."""
+    rp, ci = _i32(rowptr), _i32(colidx)
+    orp, oci = _P_INT(), _P_INT()
+    nnz = C.c_int()
+    _check(lib().pvglu_obtain_fillin(int(n), _ip(rp), _ip(ci), C.byref(orp), C.byref(oci),
+                                     C.byref(nnz), _stream(stream)))
+    try:
+        out_rp = np.ctypeslib.as_array(orp, shape=(n + 1,)).copy()
+        out_ci = (np.ctypeslib.as_array(oci, shape=(nnz.value,)).copy() if nnz.value
+                  else np.zeros(0, np.int32))
+    finally:
+        lib().pvglu_free(C.cast(orp, _VP))
+        lib().pvglu_free(C.cast(oci, _VP))
+    return out_rp.astype(np.int32), out_ci.astype(np.int32)
+
+
+def inclusive_scan_device(t, stream=None):
+    """In-place inclusive prefix sum of an int32 device tensor."""
+    _check(lib().pvglu_inclusive_scan_device(_ptr(t), int(t.numel()), _stream(stream)))
+
+
+_PV_SIG = {
+    "pvglu_pv_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(_VP)]),
+    "pvglu_pv_destroy": (None, [_VP]),
+    "pvglu_pv_info": (C.c_int, [_VP, C.c_int]),
+    "pvglu_pv_modules": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "pvglu_pv_cables": (C.c_int, [_VP, _VP, _VP, _VP]),
+    "pvglu_pv_buses": (C.c_int, [_VP, _VP]),
+    "pvglu_pv_positions": (C.c_int, [_VP, _VP, _VP]),
+    "pvglu_pv_irradiance": (C.c_int, [_VP, C.c_int64, C.c_double, _VP]),
+    "pvglu_pv_pattern": (C.c_int, [_VP, _VP, _VP]),
+    "pvglu_pv_module_current": (C.c_int, [C.c_double, C.c_double, C.c_double, _P_DBL, _P_DBL,
+                                          _P_DBL]),
+    "pvglu_amd_order": (C.c_int, [C.c_int, _P_INT, _P_INT, _P_INT]),
+    "pvglu_reorder": (C.c_int, [C.c_int, _P_INT, _P_INT, _P_INT, _P_INT]),
+    "pvglu_pv_newton_dropin": (C.c_int, [_VP, _VP, _VP, _VP, C.c_double, C.c_int, C.c_int,
+                                         _VP]),
+    "pvglu_pv_newton_batched": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, _VP, C.c_double,
+                                          C.c_int, C.c_int, _VP]),
+    "pvglu_pv_gradients": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, _VP, _VP, C.c_double,
+                                     C.c_int]),
+    "pvglu_pv_plan": (_VP, [_VP]),
+    "pvglu_pv_stamp": (C.c_int, [_VP, C.c_int, _VP, _VP, C.c_double, _VP, _VP, _VP]),
+}
+
+
+def _pvlib():
+    L = lib()
+    if not getattr(L, "_pv_sig_done", False):
+        for name, (res, args) in _PV_SIG.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        L._pv_sig_done = True
+    return L
+
+
+def _dp(a):
+    return a.ctypes.data_as(_VP) if a is not None else None
+
+
+def amd_order(n, rowptr, colidx):
+    """AMD 1.1 ordering restated in the library (``pvglu_amd_order``)."""
+    rp, ci = _i32(rowptr), _i32(colidx)
+    P = np.zeros(n, np.int32)
+    _check(_pvlib().pvglu_amd_order(int(n), _ip(rp), _ip(ci), _ip(P)))
+    return P
+
+
+def reorder(n, rowptr, colidx):
+    """reorder_matrix: AMD permutation P and the 004b row swaps Prow."""
+    rp, ci = _i32(rowptr), _i32(colidx)
+    P = np.zeros(n, np.int32)
+    Prow = np.zeros(n, np.int32)
+    _check(_pvlib().pvglu_reorder(int(n), _ip(rp), _ip(ci), _ip(P), _ip(Prow)))
+    return P, Prow
+
+
+def module_current(V, Iph, rsh):
+    I, dV, dI = C.c_double(), C.c_double(), C.c_double()
+    _check(_pvlib().pvglu_pv_module_current(float(V), float(Iph), float(rsh), C.byref(I),
+                                            C.byref(dV), C.byref(dI)))
+    return I.value, dV.value, dI.value
+
+
+class PV:
+    """A config plant: ``inverters x strings x modules`` (``pvglu_pv_create``)."""
+
+    def __init__(self, inverters, strings, modules, seed=0):
+        h = _VP()
+        _check(_pvlib().pvglu_pv_create(int(inverters), int(strings), int(modules), int(seed),
+                                        C.byref(h)))
+        self._h = h
+        self.I, self.S, self.M = int(inverters), int(strings), int(modules)
+        L = _pvlib()
+        self.n = L.pvglu_pv_info(h, 0)
+        self.nmod = L.pvglu_pv_info(h, 1)
+        self.ncab = L.pvglu_pv_info(h, 2)
+        self.nnz_a = L.pvglu_pv_info(h, 3)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _pvlib().pvglu_pv_destroy(self._h)
+            self._h = None
+
+    __del__ = close
+
+    def modules(self):
+        pos = np.zeros(self.nmod, np.int32)
+        neg = np.zeros(self.nmod, np.int32)
+        rsh = np.zeros(self.nmod)
+        _check(_pvlib().pvglu_pv_modules(self._h, _dp(pos), _dp(neg), _dp(rsh)))
+        return pos, neg, rsh
+
+    def cables(self):
+        a = np.zeros(self.ncab, np.int32)
+        b = np.zeros(self.ncab, np.int32)
+        g = np.zeros(self.ncab)
+        _check(_pvlib().pvglu_pv_cables(self._h, _dp(a), _dp(b), _dp(g)))
+        return a, b, g
+
+    def buses(self):
+        bus = np.zeros(self.I, np.int32)
+        _check(_pvlib().pvglu_pv_buses(self._h, _dp(bus)))
+        return bus
+
+    def positions(self):
+        x = np.zeros(self.nmod)
+        y = np.zeros(self.nmod)
+        _check(_pvlib().pvglu_pv_positions(self._h, _dp(x), _dp(y)))
+        return x, y
+
+    def irradiance(self, scenario=-1, t=0.0):
+        g = np.zeros(self.nmod)
+        _check(_pvlib().pvglu_pv_irradiance(self._h, int(scenario), float(t), _dp(g)))
+        return g
+
+    def pattern(self):
+        rp = np.zeros(self.n + 1, np.int32)
+        ci = np.zeros(self.nnz_a, np.int32)
+        _check(_pvlib().pvglu_pv_pattern(self._h, _dp(rp), _dp(ci)))
+        return rp, ci
+
+    def plan(self):
+        """The plant's symbolic plan (a non-owning Plan view)."""
+        h = _pvlib().pvglu_pv_plan(self._h)
+        if not h:
+            msg = lib().pvglu_last_error()
+            raise PvgluError(E_ARG, msg.decode() if msg else "pv_plan failed")
+        p = Plan.__new__(Plan)
+        p._h = None            # not owned: the plant destroys it
+        p._view = C.c_void_p(h)
+        p.n = self.n
+        p._owner = self
+        return p
+
+    def stamp(self, batch, g_dev, v_dev, a_dev, b_dev, g_virtual=-1.0, stream=None):
+        """Device stamping of the companion systems (slot-major layouts)."""
+        _check(_pvlib().pvglu_pv_stamp(self._h, int(batch), _ptr(g_dev), _ptr(v_dev),
+                                       float(g_virtual), _ptr(a_dev), _ptr(b_dev),
+                                       _stream(stream)))
+
+    def v0(self, batch=None):
+        v = np.full(self.nmod, 600.0 / self.M)
+        return v if batch is None else np.repeat(v[:, None], batch, axis=1).copy()
+
+    def newton_dropin(self, g, v=None, tol=1e-10, ladder=True, fixed_iters=0):
+        """Reference call sequence, one instance.  Returns (x, v, stats)."""
+        g = np.ascontiguousarray(g, np.float64)
+        v = self.v0() if v is None else np.ascontiguousarray(v, np.float64).copy()
+        x = np.zeros(self.n)
+        stats = np.zeros(4)
+        _check(_pvlib().pvglu_pv_newton_dropin(self._h, _dp(g), _dp(v), _dp(x), float(tol),
+                                               int(bool(ladder)), int(fixed_iters), _dp(stats)))
+        return x, v, stats
+
+    def newton_batched(self, g, v=None, tol=1e-10, max_iters=5000, ladder=True):
+        """g, v: [module][instance].  Returns (x [node][inst], v, power, stats)."""
+        g = np.ascontiguousarray(g, np.float64)
+        B = g.shape[1]
+        v = self.v0(B) if v is None else np.ascontiguousarray(v, np.float64).copy()
+        x = np.zeros((self.n, B))
+        p = np.zeros(B)
+        stats = np.zeros(4)
+        _check(_pvlib().pvglu_pv_newton_batched(self._h, B, _dp(g), _dp(v), _dp(x), _dp(p),
+                                                float(tol), int(max_iters), int(bool(ladder)),
+                                                _dp(stats)))
+        return x, v, p, stats
+
+    def gradients(self, g, v=None, tol=1e-12, max_iters=5000):
+        """Adjoint dP/dG [module][inst] and dP/dg_cable [cable][inst]."""
+        g = np.ascontiguousarray(g, np.float64)
+        B = g.shape[1]
+        v = self.v0(B) if v is None else np.ascontiguousarray(v, np.float64).copy()
+        p = np.zeros(B)
+        dG = np.zeros((self.nmod, B))
+        dgc = np.zeros((self.ncab, B))
+        _check(_pvlib().pvglu_pv_gradients(self._h, B, _dp(g), _dp(v), _dp(p), _dp(dG),
+                                           _dp(dgc), float(tol), int(max_iters)))
+        return p, dG, dgc, v

@@ -1,0 +1,866 @@
+// pvglu_device.hip -- gfx950 kernels and the device half of the C-ABI.
+//
+// Numeric path (the Newton-step refactor + solve of the reference,
+// PV/factorize_electrical_system_CUDA.cu:61-152, PV/factorize_matrix_GLU_CUDA.cuh:51-211,
+// PV/triangular_substitution.cu:61-271), written in gather form:
+//   * a lane owns one (column, instance) and computes each entry of the column
+//     as  a(i,c) - sum_j L(i,j) U(j,c)  with j ascending, reading operands
+//     that earlier levels (or earlier columns of its chain) finished, so no
+//     atomics are needed and every entry sees the exact operation order of the
+//     sequential CPU GLU (bit-identical results);
+//   * values are slot-major / instance-minor: with `width` lanes of one chain
+//     on consecutive instances, a wavefront reads 64 x 8 B = 512 B contiguous;
+//     when width == 64 all pattern metadata is wavefront-uniform and lives in
+//     scalar registers (readfirstlane), so pattern reads are s_load traffic;
+//   * LEVEL schedule: one launch per level (the reference's launch structure);
+//     COMPONENT schedule: one launch; each workgroup owns one independent
+//     component x instance tile and walks all levels with workgroup barriers.
+#include <hip/hip_runtime.h>
+
+#include "pvglu_internal.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <vector>
+
+using namespace pvglu;
+
+#define HIP_TRY(expr)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail(PVGLU_E_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                         \
+  } while (0)
+
+namespace pvglu {
+
+// Device mirror of a level set in kernel order.
+struct DevLevels {
+  int *level_chain_ptr = nullptr, *chain_col_ptr = nullptr, *cols = nullptr;
+  int *comp_ptr = nullptr, *comp_lv_begin = nullptr, *comp_lv_end = nullptr;
+  int nlev = 0, max_comp_chains = 0;
+  std::vector<int> h_level_chain_ptr; // host copy for launch sizing
+};
+
+struct DevicePlan {
+  int device = -1;
+  int *cp = nullptr, *ri = nullptr, *op_ptr = nullptr, *op_l = nullptr,
+      *op_u = nullptr, *diag_slot = nullptr, *lrow_ptr = nullptr,
+      *lrow_slot = nullptr, *lrow_col = nullptr, *urow_ptr = nullptr,
+      *urow_slot = nullptr, *urow_col = nullptr;
+  DevLevels lu, l, u, lt;
+  int *status = nullptr; // [code, column, instance, pad]
+  std::vector<void *> allocs;
+};
+
+} // namespace pvglu
+
+// ------------------------------------------------------------ kernel views
+struct Pat {
+  const int *__restrict__ cp;
+  const int *__restrict__ ri;
+  const int *__restrict__ op_ptr;
+  const int *__restrict__ op_l;
+  const int *__restrict__ op_u;
+  const int *__restrict__ diag;
+  const int *__restrict__ lrow_ptr;
+  const int *__restrict__ lrow_slot;
+  const int *__restrict__ lrow_col;
+  const int *__restrict__ urow_ptr;
+  const int *__restrict__ urow_slot;
+  const int *__restrict__ urow_col;
+  int *status;
+};
+
+struct Lev {
+  const int *__restrict__ level_chain_ptr;
+  const int *__restrict__ chain_col_ptr;
+  const int *__restrict__ cols;
+  const int *__restrict__ comp_ptr;
+  const int *__restrict__ comp_lv_begin;
+  const int *__restrict__ comp_lv_end;
+};
+
+static Pat pat_of(const DevicePlan *d) {
+  return Pat{d->cp,       d->ri,        d->op_ptr,    d->op_l,     d->op_u,
+             d->diag_slot, d->lrow_ptr, d->lrow_slot, d->lrow_col, d->urow_ptr,
+             d->urow_slot, d->urow_col, d->status};
+}
+static Lev lev_of(const DevLevels &L) {
+  return Lev{L.level_chain_ptr, L.chain_col_ptr, L.cols,
+             L.comp_ptr,        L.comp_lv_begin, L.comp_lv_end};
+}
+
+template <int W> __device__ __forceinline__ int uniform(int v) {
+  if constexpr (W == 64)
+    return __builtin_amdgcn_readfirstlane(v);
+  else
+    return v;
+}
+
+__device__ __forceinline__ void record_status(int *status, int code, int col,
+                                              int inst) {
+  if (atomicCAS(status, 0, code) == 0) {
+    status[1] = col;
+    status[2] = inst;
+  }
+}
+
+// One column of the refactor for one instance (gather form), optionally
+// followed by the forward substitution of row c.
+// Operation order per entry = GLU_CSC_CPU_sequential's (PV/GLU_CSC_CPU_levels_strings.cpp:32-62):
+// v - L*U for ascending j, then the division by the pivot for i > c.
+template <int W, bool FUSE>
+__device__ __forceinline__ void refactor_column(const Pat &P, int c,
+                                                const double *A, double *LU,
+                                                const double *b, double *x,
+                                                size_t B, int inst) {
+  const int q0 = uniform<W>(P.cp[c]);
+  const int q1 = uniform<W>(P.cp[c + 1]);
+  const int qd = uniform<W>(P.diag[c]);
+  double piv = 1.0;
+  for (int q = q0; q < q1; ++q) {
+    double v = A[(size_t)q * B + inst];
+    const int o0 = uniform<W>(P.op_ptr[q]);
+    const int o1 = uniform<W>(P.op_ptr[q + 1]);
+    for (int o = o0; o < o1; ++o) {
+      const int sl = uniform<W>(P.op_l[o]);
+      const int su = uniform<W>(P.op_u[o]);
+      v = v - LU[(size_t)sl * B + inst] * LU[(size_t)su * B + inst];
+    }
+    if (q == qd) {
+      piv = v;
+      if (fabs(v) < 1e-8)
+        record_status(P.status, PVGLU_E_SINGULAR, c, inst);
+    } else if (q > qd) {
+      v = v / piv;
+    }
+    LU[(size_t)q * B + inst] = v;
+  }
+  if constexpr (FUSE) {
+    double s = b[(size_t)c * B + inst];
+    const int k0 = uniform<W>(P.lrow_ptr[c]);
+    const int k1 = uniform<W>(P.lrow_ptr[c + 1]);
+    for (int k = k0; k < k1; ++k) {
+      const int sl = uniform<W>(P.lrow_slot[k]);
+      const int j = uniform<W>(P.lrow_col[k]);
+      s = s - LU[(size_t)sl * B + inst] * x[(size_t)j * B + inst];
+    }
+    x[(size_t)c * B + inst] = s;
+  }
+}
+
+// Row i of the forward substitution: y_i = b_i - sum_j L(i,j) y_j (j asc),
+// the per-target order of Process_Column_L_CSC_CPU sweeps (PV/GLU_CSC_CPU_Triangular.cpp:3-11,27-33).
+template <int W>
+__device__ __forceinline__ void lsolve_row(const Pat &P, int i,
+                                           const double *LU, const double *b,
+                                           double *x, size_t B, int inst) {
+  double s = b[(size_t)i * B + inst];
+  const int k0 = uniform<W>(P.lrow_ptr[i]);
+  const int k1 = uniform<W>(P.lrow_ptr[i + 1]);
+  for (int k = k0; k < k1; ++k) {
+    const int sl = uniform<W>(P.lrow_slot[k]);
+    const int j = uniform<W>(P.lrow_col[k]);
+    s = s - LU[(size_t)sl * B + inst] * x[(size_t)j * B + inst];
+  }
+  x[(size_t)i * B + inst] = s;
+}
+
+// Row i of the backward substitution: x_i = (y_i - sum_j U(i,j) x_j)/U(i,i)
+// with j descending, the per-target order of Process_Column_U_CSC_CPU sweeps
+// (PV/GLU_CSC_CPU_Triangular.cpp:13-25,35-39).
+template <int W>
+__device__ __forceinline__ void usolve_row(const Pat &P, int i,
+                                           const double *LU, double *x,
+                                           size_t B, int inst) {
+  double s = x[(size_t)i * B + inst];
+  const int k0 = uniform<W>(P.urow_ptr[i]);
+  const int k1 = uniform<W>(P.urow_ptr[i + 1]);
+  for (int k = k0; k < k1; ++k) {
+    const int su = uniform<W>(P.urow_slot[k]);
+    const int j = uniform<W>(P.urow_col[k]);
+    s = s - LU[(size_t)su * B + inst] * x[(size_t)j * B + inst];
+  }
+  const int d = uniform<W>(P.diag[i]);
+  x[(size_t)i * B + inst] = s / LU[(size_t)d * B + inst];
+}
+
+// Adjoint, U^T z = g: z_j = (g_j - sum_{i<j} U(i,j) z_i) / U(j,j), the CSC
+// column j above its diagonal (contiguous slots), i ascending.
+template <int W>
+__device__ __forceinline__ void utsolve_col(const Pat &P, int j,
+                                            const double *LU, const double *g,
+                                            double *y, size_t B, int inst) {
+  const int q0 = uniform<W>(P.cp[j]);
+  const int d = uniform<W>(P.diag[j]);
+  double s = g[(size_t)j * B + inst];
+  for (int q = q0; q < d; ++q) {
+    const int i = uniform<W>(P.ri[q]);
+    s = s - LU[(size_t)q * B + inst] * y[(size_t)i * B + inst];
+  }
+  y[(size_t)j * B + inst] = s / LU[(size_t)d * B + inst];
+}
+
+// Adjoint, L^T y = z: y_j = z_j - sum_{i>j} L(i,j) y_i, column j below its
+// diagonal, i ascending.
+template <int W>
+__device__ __forceinline__ void ltsolve_col(const Pat &P, int j,
+                                            const double *LU, double *y,
+                                            size_t B, int inst) {
+  const int d = uniform<W>(P.diag[j]);
+  const int q1 = uniform<W>(P.cp[j + 1]);
+  double s = y[(size_t)j * B + inst];
+  for (int q = d + 1; q < q1; ++q) {
+    const int i = uniform<W>(P.ri[q]);
+    s = s - LU[(size_t)q * B + inst] * y[(size_t)i * B + inst];
+  }
+  y[(size_t)j * B + inst] = s;
+}
+
+enum Op { OP_REFACTOR = 0, OP_REFACTOR_FUSED, OP_LSOLVE, OP_USOLVE, OP_UT, OP_LT };
+
+template <int W, int OP>
+__device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
+                                          const double *A, double *LU,
+                                          const double *b, double *x, size_t B,
+                                          int inst) {
+  const int t0 = uniform<W>(L.chain_col_ptr[chain]);
+  const int t1 = uniform<W>(L.chain_col_ptr[chain + 1]);
+  for (int t = t0; t < t1; ++t) {
+    const int c = uniform<W>(L.cols[t]);
+    if constexpr (OP == OP_REFACTOR)
+      refactor_column<W, false>(P, c, A, LU, b, x, B, inst);
+    else if constexpr (OP == OP_REFACTOR_FUSED)
+      refactor_column<W, true>(P, c, A, LU, b, x, B, inst);
+    else if constexpr (OP == OP_LSOLVE)
+      lsolve_row<W>(P, c, LU, b, x, B, inst);
+    else if constexpr (OP == OP_USOLVE)
+      usolve_row<W>(P, c, LU, x, B, inst);
+    else if constexpr (OP == OP_UT)
+      utsolve_col<W>(P, c, LU, b, x, B, inst);
+    else
+      ltsolve_col<W>(P, c, LU, x, B, inst);
+  }
+}
+
+// LEVEL schedule: one launch per level.  Thread -> (chain group, lane);
+// group -> (instance tile, chain of the level).
+template <int W, int OP>
+__global__ void __launch_bounds__(256)
+    k_level(Pat P, Lev L, int level, int nchains, int tiles, const double *A,
+            double *LU, const double *b, double *x, int B) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int group = uniform<W>(gid / W);
+  const int lane = gid % W;
+  if (group >= nchains * tiles)
+    return;
+  const int tile = group / nchains;
+  const int chain = uniform<W>(L.level_chain_ptr[level]) + group % nchains;
+  const int inst = tile * W + lane;
+  if (inst >= B)
+    return;
+  run_chain<W, OP>(P, L, chain, A, LU, b, x, (size_t)B, inst);
+}
+
+// COMPONENT schedule: one launch for the whole refactor + solve.  Block ->
+// (component, instance tile); each level of the component is split over the
+// block's chain slots; barriers separate levels.  Every thread reaches every
+// barrier (loop bounds are block-uniform; inactive lanes only skip work).
+template <int W, bool FUSE>
+__device__ __forceinline__ void component_levels(const Pat &P, const Lev &L,
+                                                 int comp, int slot, int slots,
+                                                 bool active, const double *A,
+                                                 double *LU, const double *b,
+                                                 double *x, size_t B, int inst,
+                                                 int op) {
+  const int e0 = L.comp_ptr[comp], e1 = L.comp_ptr[comp + 1];
+  for (int e = e0; e < e1; ++e) {
+    const int c0 = L.comp_lv_begin[e], c1 = L.comp_lv_end[e];
+    if (active)
+      for (int ch = c0 + slot; ch < c1; ch += slots) {
+        const int chain = uniform<W>(ch);
+        if (op == OP_REFACTOR_FUSED)
+          run_chain<W, OP_REFACTOR_FUSED>(P, L, chain, A, LU, b, x, B, inst);
+        else if (op == OP_REFACTOR)
+          run_chain<W, OP_REFACTOR>(P, L, chain, A, LU, b, x, B, inst);
+        else if (op == OP_LSOLVE)
+          run_chain<W, OP_LSOLVE>(P, L, chain, A, LU, b, x, B, inst);
+        else if (op == OP_USOLVE)
+          run_chain<W, OP_USOLVE>(P, L, chain, A, LU, b, x, B, inst);
+        else if (op == OP_UT)
+          run_chain<W, OP_UT>(P, L, chain, A, LU, b, x, B, inst);
+        else
+          run_chain<W, OP_LT>(P, L, chain, A, LU, b, x, B, inst);
+      }
+    __syncthreads();
+  }
+}
+
+// mode 0: refactor only; 1: refactor + solve; 2: solve only; 3: transposed solve
+template <int W>
+__global__ void __launch_bounds__(1024)
+    k_component(Pat P, Lev Llu, Lev Ll, Lev Lu, Lev Llt, int mode, bool fuse,
+                int tiles, const double *A, double *LU, const double *b,
+                double *x, int B) {
+  const int comp = blockIdx.x / tiles;
+  const int tile = blockIdx.x % tiles;
+  const int slots = blockDim.x / W;
+  const int slot = uniform<W>((int)threadIdx.x / W);
+  const int lane = threadIdx.x % W;
+  const int inst = tile * W + lane;
+  const bool active = inst < B;
+  const size_t Bs = (size_t)B;
+  if (mode == 3) {
+    component_levels<W, false>(P, Llu, comp, slot, slots, active, A, LU, b, x,
+                               Bs, inst, OP_UT);
+    component_levels<W, false>(P, Llt, comp, slot, slots, active, A, LU, b, x,
+                               Bs, inst, OP_LT);
+    return;
+  }
+  if (mode <= 1)
+    component_levels<W, false>(P, Llu, comp, slot, slots, active, A, LU, b, x,
+                               Bs, inst,
+                               (mode == 1 && fuse) ? OP_REFACTOR_FUSED
+                                                   : OP_REFACTOR);
+  if (mode == 0)
+    return;
+  if (mode == 2 || !fuse)
+    component_levels<W, false>(P, Ll, comp, slot, slots, active, A, LU, b, x,
+                               Bs, inst, OP_LSOLVE);
+  component_levels<W, false>(P, Lu, comp, slot, slots, active, A, LU, b, x, Bs,
+                             inst, OP_USOLVE);
+}
+
+__global__ void k_fill_int(int *p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    p[i] = v;
+}
+
+// ----------------------------------------------------------- 011 update
+// One thread per (element chunk, instance).  Per element the damping factor
+// halves until |candidate - old| <= 1 V and nothing is NaN
+// (is_value_change_justified, PV/011_calculate_new_iteration.cpp:6-17); the
+// reference's loop halves each unjustified element's factor independently and
+// aborts once any factor is below 1e-13 (:44-74).
+__global__ void __launch_bounds__(256)
+    k_newton_update(int ne, int B, int per_thread, const int *__restrict__ pos,
+                    const int *__restrict__ neg, const double *__restrict__ x,
+                    double *__restrict__ v, double *delta_sum,
+                    unsigned long long *delta_max, int *status) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long total = (long)((ne + per_thread - 1) / per_thread) * B;
+  if (gid >= total)
+    return;
+  const int in = (int)(gid % B);
+  const int chunk = (int)(gid / B);
+  const int e0 = chunk * per_thread;
+  const int e1 = min(ne, e0 + per_thread);
+  double sum = 0.0, mx = 0.0;
+  for (int e = e0; e < e1; ++e) {
+    const int p = pos[e], m_ = neg[e];
+    const double vp = p >= 0 ? x[(size_t)p * B + in] : 0.0;
+    const double vn = m_ >= 0 ? x[(size_t)m_ * B + in] : 0.0;
+    const double nw = vp - vn; // voltage_drop_dc (PV/002_base_electric_system.cpp:21-33)
+    const double old = v[(size_t)e * B + in];
+    double m = 1.0, cand = old;
+    bool ok = false;
+    for (int k = 0; k < 64; ++k) {
+      if (m < 1e-13)
+        break;
+      cand = old * (1.0 - m) + nw * m;
+      if (!isnan(old) && !isnan(cand) && !(fabs(old - cand) > 1.0)) {
+        ok = true;
+        break;
+      }
+      m = m / 2.0;
+    }
+    if (!ok) {
+      record_status(status, PVGLU_E_NOCONV, e, in);
+      cand = old;
+    }
+    const double d = fabs(cand - old);
+    sum += d;
+    mx = fmax(mx, d);
+    v[(size_t)e * B + in] = cand;
+  }
+  atomicAdd(&delta_sum[in], sum);
+  atomicMax(&delta_max[in], (unsigned long long)__double_as_longlong(mx));
+}
+
+// -------------------------------------------------------------- host side
+
+namespace pvglu {
+
+template <typename T>
+static int dev_upload(DevicePlan *d, T **dst, const std::vector<T> &src) {
+  size_t bytes = sizeof(T) * std::max<size_t>(src.size(), 1);
+  void *p = nullptr;
+  HIP_TRY(hipMalloc(&p, bytes));
+  d->allocs.push_back(p);
+  if (!src.empty())
+    HIP_TRY(hipMemcpy(p, src.data(), sizeof(T) * src.size(),
+                      hipMemcpyHostToDevice));
+  *dst = (T *)p;
+  return PVGLU_OK;
+}
+
+static int upload_levels(DevicePlan *d, const KernelLevels &k, DevLevels &L) {
+  int st;
+  if ((st = dev_upload(d, &L.level_chain_ptr, k.level_chain_ptr)) ||
+      (st = dev_upload(d, &L.chain_col_ptr, k.chain_col_ptr)) ||
+      (st = dev_upload(d, &L.cols, k.cols)) ||
+      (st = dev_upload(d, &L.comp_ptr, k.comp_ptr)) ||
+      (st = dev_upload(d, &L.comp_lv_begin, k.comp_lv_begin)) ||
+      (st = dev_upload(d, &L.comp_lv_end, k.comp_lv_end)))
+    return st;
+  L.nlev = (int)k.level_chain_ptr.size() - 1;
+  L.max_comp_chains = k.max_comp_chains;
+  L.h_level_chain_ptr = k.level_chain_ptr;
+  return PVGLU_OK;
+}
+
+void device_plan_destroy(pvglu_plan *plan) {
+  if (!plan->dev)
+    return;
+  int cur = 0;
+  bool switched = false;
+  if (hipGetDevice(&cur) == hipSuccess && cur != plan->dev->device) {
+    switched = hipSetDevice(plan->dev->device) == hipSuccess;
+  }
+  for (void *p : plan->dev->allocs)
+    (void)hipFree(p);
+  if (switched)
+    (void)hipSetDevice(cur);
+  delete plan->dev;
+  plan->dev = nullptr;
+}
+
+} // namespace pvglu
+
+static int require_device(pvglu_plan *P) {
+  if (!P)
+    return fail(PVGLU_E_ARG, "null plan");
+  if (!P->dev) {
+    int st = pvglu_plan_upload(P);
+    if (st)
+      return st;
+  }
+  return PVGLU_OK;
+}
+
+extern "C" int pvglu_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess)
+    return 0;
+  return n;
+}
+
+extern "C" const char *pvglu_version(void) { return "pvglu 1.0 gfx950"; }
+
+extern "C" int pvglu_plan_upload(pvglu_plan *P) {
+  clear_error();
+  if (!P)
+    return fail(PVGLU_E_ARG, "plan_upload: null plan");
+  if (P->dev)
+    return PVGLU_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(PVGLU_E_HIP, "plan_upload: no HIP device");
+  auto *d = new DevicePlan();
+  HIP_TRY(hipGetDevice(&d->device));
+  P->dev = d;
+  int st;
+  if ((st = dev_upload(d, &d->cp, P->cp)) || (st = dev_upload(d, &d->ri, P->ri)) ||
+      (st = dev_upload(d, &d->op_ptr, P->op_ptr)) ||
+      (st = dev_upload(d, &d->op_l, P->op_l)) ||
+      (st = dev_upload(d, &d->op_u, P->op_u)) ||
+      (st = dev_upload(d, &d->diag_slot, P->diag_slot)) ||
+      (st = dev_upload(d, &d->lrow_ptr, P->lrow_ptr)) ||
+      (st = dev_upload(d, &d->lrow_slot, P->lrow_slot)) ||
+      (st = dev_upload(d, &d->lrow_col, P->lrow_col)) ||
+      (st = dev_upload(d, &d->urow_ptr, P->urow_ptr)) ||
+      (st = dev_upload(d, &d->urow_slot, P->urow_slot)) ||
+      (st = dev_upload(d, &d->urow_col, P->urow_col)) ||
+      (st = upload_levels(d, P->k_lu, d->lu)) ||
+      (st = upload_levels(d, P->k_l, d->l)) ||
+      (st = upload_levels(d, P->k_u, d->u)) ||
+      (st = upload_levels(d, P->k_lt, d->lt)) ||
+      (st = dev_upload(d, &d->status, std::vector<int>(4, 0)))) {
+    device_plan_destroy(P);
+    return st;
+  }
+  return PVGLU_OK;
+}
+
+// ------------------------------------------------------------- launching
+
+static bool valid_width(int w) {
+  return w == 1 || w == 2 || w == 4 || w == 8 || w == 16 || w == 32 || w == 64;
+}
+
+static void pick_default_schedule(pvglu_plan *P, int B, int &variant, int &W,
+                                  int &threads) {
+  variant = P->sched_variant;
+  W = P->sched_width;
+  if (!valid_width(W))
+    W = B >= 64 ? 64 : (B >= 16 ? 16 : (B >= 4 ? 4 : 1));
+  if (variant == PVGLU_SCHED_AUTO)
+    variant = (P->ncomp >= 8 && B > 1) ? PVGLU_SCHED_COMPONENT
+                                       : PVGLU_SCHED_LEVEL;
+  // COMPONENT: chain slots per block = widest component level, capped so the
+  // block has at most 1024 threads.
+  int maxc = std::max({P->k_lu.max_comp_chains, P->k_u.max_comp_chains,
+                       P->k_l.max_comp_chains, P->k_lt.max_comp_chains, 1});
+  int slots = 1;
+  while (slots < maxc && slots * 2 * W <= 1024)
+    slots *= 2;
+  threads = std::max(64, slots * W);
+  if (threads > 1024)
+    threads = 1024;
+}
+
+template <int W, int OP>
+static int launch_levels_w(pvglu_plan *P, const DevLevels &L, const double *A,
+                           double *LU, const double *b, double *x, int B,
+                           hipStream_t s) {
+  const int tiles = (B + W - 1) / W;
+  const Pat pat = pat_of(P->dev);
+  const Lev lev = lev_of(L);
+  for (int lv = 0; lv < L.nlev; ++lv) {
+    const int nch = L.h_level_chain_ptr[lv + 1] - L.h_level_chain_ptr[lv];
+    const long threads = (long)nch * tiles * W;
+    const int blocks = (int)((threads + 255) / 256);
+    if (blocks == 0)
+      continue;
+    hipLaunchKernelGGL((k_level<W, OP>), dim3(blocks), dim3(256), 0, s, pat,
+                       lev, lv, nch, tiles, A, LU, b, x, B);
+  }
+  HIP_TRY(hipGetLastError());
+  return PVGLU_OK;
+}
+
+template <int OP>
+static int launch_levels(pvglu_plan *P, const DevLevels &L, int W,
+                         const double *A, double *LU, const double *b,
+                         double *x, int B, hipStream_t s) {
+  switch (W) {
+  case 1: return launch_levels_w<1, OP>(P, L, A, LU, b, x, B, s);
+  case 2: return launch_levels_w<2, OP>(P, L, A, LU, b, x, B, s);
+  case 4: return launch_levels_w<4, OP>(P, L, A, LU, b, x, B, s);
+  case 8: return launch_levels_w<8, OP>(P, L, A, LU, b, x, B, s);
+  case 16: return launch_levels_w<16, OP>(P, L, A, LU, b, x, B, s);
+  case 32: return launch_levels_w<32, OP>(P, L, A, LU, b, x, B, s);
+  default: return launch_levels_w<64, OP>(P, L, A, LU, b, x, B, s);
+  }
+}
+
+template <int W>
+static int launch_component_w(pvglu_plan *P, int mode, int threads,
+                              const double *A, double *LU, const double *b,
+                              double *x, int B, hipStream_t s) {
+  const int tiles = (B + W - 1) / W;
+  const DevicePlan *d = P->dev;
+  hipLaunchKernelGGL((k_component<W>), dim3(P->ncomp * tiles), dim3(threads),
+                     0, s, pat_of(d), lev_of(d->lu), lev_of(d->l),
+                     lev_of(d->u), lev_of(d->lt), mode, P->fuse_lsolve, tiles,
+                     A, LU, b, x, B);
+  HIP_TRY(hipGetLastError());
+  return PVGLU_OK;
+}
+
+static int launch_component(pvglu_plan *P, int W, int mode, int threads,
+                            const double *A, double *LU, const double *b,
+                            double *x, int B, hipStream_t s) {
+  switch (W) {
+  case 1: return launch_component_w<1>(P, mode, threads, A, LU, b, x, B, s);
+  case 2: return launch_component_w<2>(P, mode, threads, A, LU, b, x, B, s);
+  case 4: return launch_component_w<4>(P, mode, threads, A, LU, b, x, B, s);
+  case 8: return launch_component_w<8>(P, mode, threads, A, LU, b, x, B, s);
+  case 16: return launch_component_w<16>(P, mode, threads, A, LU, b, x, B, s);
+  case 32: return launch_component_w<32>(P, mode, threads, A, LU, b, x, B, s);
+  default: return launch_component_w<64>(P, mode, threads, A, LU, b, x, B, s);
+  }
+}
+
+// mode: 0 refactor, 1 refactor+solve, 2 solve, 3 transposed solve
+static int run_numeric(pvglu_plan *P, int mode, const double *A, double *LU,
+                       const double *b, double *x, int B, void *stream) {
+  clear_error();
+  int st = require_device(P);
+  if (st)
+    return st;
+  if (B <= 0)
+    return fail(PVGLU_E_ARG, "batch must be positive");
+  if ((mode != 2 && mode != 3 && (!A || !LU)) ||
+      ((mode == 2 || mode == 3) && !LU) || (mode >= 1 && (!b || !x)))
+    return fail(PVGLU_E_ARG, "null device pointer");
+  hipStream_t s = (hipStream_t)stream;
+  int variant, W, threads;
+  pick_default_schedule(P, B, variant, W, threads);
+  if (variant == PVGLU_SCHED_COMPONENT)
+    return launch_component(P, W, mode, threads, A, LU, b, x, B, s);
+  DevicePlan *d = P->dev;
+  if (mode == 3) {
+    if ((st = launch_levels<OP_UT>(P, d->lu, W, nullptr, (double *)LU, b, x,
+                                   B, s)))
+      return st;
+    return launch_levels<OP_LT>(P, d->lt, W, nullptr, (double *)LU, nullptr,
+                                x, B, s);
+  }
+  if (mode <= 1) {
+    bool fused = mode == 1 && P->fuse_lsolve;
+    st = fused ? launch_levels<OP_REFACTOR_FUSED>(P, d->lu, W, A, LU, b, x, B, s)
+               : launch_levels<OP_REFACTOR>(P, d->lu, W, A, LU, b, x, B, s);
+    if (st || mode == 0)
+      return st;
+    if (!fused && (st = launch_levels<OP_LSOLVE>(P, d->l, W, nullptr, LU, b,
+                                                 x, B, s)))
+      return st;
+    return launch_levels<OP_USOLVE>(P, d->u, W, nullptr, LU, nullptr, x, B, s);
+  }
+  if ((st = launch_levels<OP_LSOLVE>(P, d->l, W, nullptr, (double *)LU, b, x,
+                                     B, s)))
+    return st;
+  return launch_levels<OP_USOLVE>(P, d->u, W, nullptr, (double *)LU, nullptr,
+                                  x, B, s);
+}
+
+extern "C" int pvglu_refactor(pvglu_plan *P, const double *a, double *lu,
+                              int batch, void *stream) {
+  return run_numeric(P, 0, a, lu, nullptr, nullptr, batch, stream);
+}
+
+extern "C" int pvglu_solve(pvglu_plan *P, const double *lu, const double *b,
+                           double *x, int batch, void *stream) {
+  return run_numeric(P, 2, nullptr, (double *)lu, b, x, batch, stream);
+}
+
+extern "C" int pvglu_refactor_solve(pvglu_plan *P, const double *a, double *lu,
+                                    const double *b, double *x, int batch,
+                                    void *stream) {
+  return run_numeric(P, 1, a, lu, b, x, batch, stream);
+}
+
+extern "C" int pvglu_solve_transpose(pvglu_plan *P, const double *lu,
+                                     const double *g, double *y, int batch,
+                                     void *stream) {
+  return run_numeric(P, 3, nullptr, (double *)lu, g, y, batch, stream);
+}
+
+extern "C" int pvglu_status(pvglu_plan *P, int *code, int *column,
+                            int *instance) {
+  clear_error();
+  int st = require_device(P);
+  if (st)
+    return st;
+  int h[4] = {0, 0, 0, 0};
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(h, P->dev->status, sizeof h, hipMemcpyDeviceToHost));
+  if (code)
+    *code = h[0];
+  if (column)
+    *column = h[1];
+  if (instance)
+    *instance = h[2];
+  return PVGLU_OK;
+}
+
+extern "C" int pvglu_status_reset(pvglu_plan *P, void *stream) {
+  clear_error();
+  int st = require_device(P);
+  if (st)
+    return st;
+  hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, (hipStream_t)stream,
+                     P->dev->status, 4, 0);
+  HIP_TRY(hipGetLastError());
+  return PVGLU_OK;
+}
+
+extern "C" int pvglu_set_schedule(pvglu_plan *P, int variant, int width) {
+  clear_error();
+  if (!P)
+    return fail(PVGLU_E_ARG, "set_schedule: null plan");
+  if (variant < PVGLU_SCHED_AUTO || variant > PVGLU_SCHED_COMPONENT)
+    return fail(PVGLU_E_ARG, "set_schedule: unknown variant %d", variant);
+  if (width != 0 && !valid_width(width))
+    return fail(PVGLU_E_ARG, "set_schedule: width %d not a power of two <= 64",
+                width);
+  P->sched_variant = variant;
+  P->sched_width = width;
+  return PVGLU_OK;
+}
+
+extern "C" int pvglu_get_schedule(const pvglu_plan *P, int *variant,
+                                  int *width) {
+  if (!P)
+    return fail(PVGLU_E_ARG, "get_schedule: null plan");
+  if (variant)
+    *variant = P->sched_variant;
+  if (width)
+    *width = P->sched_width;
+  return PVGLU_OK;
+}
+
+extern "C" int pvglu_newton_update(pvglu_plan *P, int ne, int batch,
+                                   const int *pos, const int *neg,
+                                   const double *x, double *v_elem,
+                                   double *delta_sum, double *delta_max,
+                                   void *stream) {
+  clear_error();
+  int st = require_device(P);
+  if (st)
+    return st;
+  if (ne <= 0 || batch <= 0 || !pos || !neg || !x || !v_elem || !delta_sum ||
+      !delta_max)
+    return fail(PVGLU_E_ARG, "newton_update: bad arguments");
+  const int per = 32;
+  const long total = (long)((ne + per - 1) / per) * batch;
+  const int blocks = (int)((total + 255) / 256);
+  // zero the per-instance reductions (fill kernel: capture-safe)
+  hipLaunchKernelGGL(k_fill_int, dim3((2 * batch + 255) / 256), dim3(256), 0,
+                     (hipStream_t)stream, (int *)delta_sum, 2 * batch, 0);
+  hipLaunchKernelGGL(k_fill_int, dim3((2 * batch + 255) / 256), dim3(256), 0,
+                     (hipStream_t)stream, (int *)delta_max, 2 * batch, 0);
+  hipLaunchKernelGGL(k_newton_update, dim3(blocks), dim3(256), 0,
+                     (hipStream_t)stream, ne, batch, per, pos, neg, x, v_elem,
+                     delta_sum, (unsigned long long *)delta_max,
+                     P->dev->status);
+  HIP_TRY(hipGetLastError());
+  return PVGLU_OK;
+}
+
+// ------------------------------------------------------------- autotuner
+// Times every (variant, width) candidate of refactor+solve for one batch size
+// on library-allocated scratch values (diagonal 1e6, off-diagonal 1: a
+// diagonally dominant stand-in; timing does not depend on the values) and
+// keeps the fastest.  Results are cached per (pattern hash, batch).
+
+__global__ void k_tune_values(int phase, const int *cp, const int *diag, int n,
+                              double *a, double *b, int B) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (phase == 0) {
+    if (gid < (long)cp[n] * B)
+      a[gid] = 1.0;
+    if (gid < (long)n * B)
+      b[gid] = 1.0;
+  } else if (gid < (long)n * B) {
+    const int col = (int)(gid / B), inst = (int)(gid % B);
+    a[(size_t)diag[col] * B + inst] = 1e6;
+  }
+}
+
+static std::mutex g_tune_mu;
+static std::map<std::pair<uint64_t, int>, std::pair<int, int>> g_tune_cache;
+
+extern "C" int pvglu_autotune(pvglu_plan *P, int batch, int reps,
+                              double *best_ms) {
+  clear_error();
+  int st = require_device(P);
+  if (st)
+    return st;
+  if (batch <= 0)
+    return fail(PVGLU_E_ARG, "autotune: batch must be positive");
+  if (reps <= 0)
+    reps = 5;
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune_cache.find({P->hash, batch});
+    if (it != g_tune_cache.end() && !best_ms) {
+      P->sched_variant = it->second.first;
+      P->sched_width = it->second.second;
+      P->tuned_batch = batch;
+      return PVGLU_OK;
+    }
+  }
+  const size_t nv = (size_t)P->nnz * batch, nx = (size_t)P->n * batch;
+  double *a = nullptr, *lu = nullptr, *b = nullptr, *x = nullptr;
+  auto release = [&]() {
+    (void)hipFree(a);
+    (void)hipFree(lu);
+    (void)hipFree(b);
+    (void)hipFree(x);
+  };
+  if (hipMalloc(&a, nv * 8) != hipSuccess || hipMalloc(&lu, nv * 8) != hipSuccess ||
+      hipMalloc(&b, nx * 8) != hipSuccess || hipMalloc(&x, nx * 8) != hipSuccess) {
+    release();
+    return fail(PVGLU_E_HIP, "autotune: scratch allocation failed");
+  }
+  hipStream_t s;
+  if (hipStreamCreate(&s) != hipSuccess) {
+    release();
+    return fail(PVGLU_E_HIP, "autotune: stream creation failed");
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  for (int phase = 0; phase < 2; ++phase)
+    hipLaunchKernelGGL(k_tune_values, dim3((unsigned)((nv + 255) / 256)),
+                       dim3(256), 0, s, phase, P->dev->cp, P->dev->diag_slot,
+                       P->n, a, b, batch);
+  const int save_v = P->sched_variant, save_w = P->sched_width;
+  double best = 1e300;
+  int bv = PVGLU_SCHED_LEVEL, bw = 1;
+  const int widths[] = {1, 2, 4, 8, 16, 32, 64};
+  for (int variant = PVGLU_SCHED_LEVEL; variant <= PVGLU_SCHED_COMPONENT;
+       ++variant) {
+    for (int w : widths) {
+      if (w > 1 && w / 2 >= batch)
+        continue; // wider tiles only idle lanes
+      P->sched_variant = variant;
+      P->sched_width = w;
+      st = pvglu_refactor_solve(P, a, lu, b, x, batch, s);
+      if (st)
+        break;
+      std::vector<float> t;
+      for (int r = 0; r < reps; ++r) {
+        (void)hipEventRecord(e0, s);
+        st = pvglu_refactor_solve(P, a, lu, b, x, batch, s);
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        t.push_back(ms);
+      }
+      std::sort(t.begin(), t.end());
+      const double med = t[t.size() / 2];
+      if (med < best) {
+        best = med;
+        bv = variant;
+        bw = w;
+      }
+    }
+    if (st)
+      break;
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  release();
+  if (st) {
+    P->sched_variant = save_v;
+    P->sched_width = save_w;
+    return st;
+  }
+  // clear any status the stand-in values may have recorded
+  int zero[4] = {0, 0, 0, 0};
+  HIP_TRY(hipMemcpy(P->dev->status, zero, sizeof zero, hipMemcpyHostToDevice));
+  P->sched_variant = bv;
+  P->sched_width = bw;
+  P->tuned_batch = batch;
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    g_tune_cache[{P->hash, batch}] = {bv, bw};
+  }
+  if (best_ms)
+    *best_ms = best;
+  return PVGLU_OK;
+}

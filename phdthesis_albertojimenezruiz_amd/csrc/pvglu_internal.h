@@ -1,0 +1,93 @@
+// pvglu_internal.h -- internal types shared by the host symbolic code, the
+// PV plant harness and the HIP kernels.
+#pragma once
+
+#include "pvglu.h"
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace pvglu {
+
+// Thread-local error message + status helper.
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+void clear_error();
+
+// Levels of chains of columns in the flattened form of
+// PV/008_convert_chains_columns_levels_into_vector.cpp:4-30.
+struct LevelSet {
+  std::vector<int> start_levels; // nlev + 1 (offsets into chains list)
+  std::vector<int> start_chains; // nchains + 1 (offsets into cols)
+  std::vector<int> chains;       // n columns
+  int nlev() const { return (int)start_levels.size() - 1; }
+  int nchains() const { return (int)start_chains.size() - 1; }
+};
+
+// Dependency rules of PV/006_get_dependencies_GLU.cpp plus the transposed L
+// rule used by the adjoint.
+enum DepKind { DEP_GLU = 0, DEP_L = 1, DEP_U = 2, DEP_LT = 3 };
+
+// deps as CSR (ascending entries).  Host, O(nnz).
+void dependencies(DepKind kind, int n, const std::vector<int> &rp,
+                  const std::vector<int> &ci, std::vector<int> &dptr,
+                  std::vector<int> &didx);
+// 007 + 008.  Returns false on a dependency cycle.
+bool levels_with_chains(int n, const std::vector<int> &dptr,
+                        const std::vector<int> &didx, LevelSet &out);
+
+// Per-kernel view of a level set, reordered so that the chains of one
+// independent component are contiguous inside each level.
+struct KernelLevels {
+  std::vector<int> level_chain_ptr; // nlev+1, offsets into chain_col_ptr
+  std::vector<int> chain_col_ptr;   // nchains+1, offsets into cols
+  std::vector<int> cols;            // n
+  // component view: comp_ptr[c]..comp_ptr[c+1] indexes comp_lv_* entries,
+  // each a (chain_begin, chain_end) range of one level of component c.
+  std::vector<int> comp_ptr;
+  std::vector<int> comp_lv_begin, comp_lv_end;
+  int max_comp_chains = 0; // widest level of any component
+};
+
+struct DevicePlan; // defined in the HIP translation unit
+
+} // namespace pvglu
+
+struct pvglu_plan {
+  int n = 0, nnz = 0;
+  // Filled pattern, CSR and CSC views with the maps of PV/010_create_CSR_CSC.cpp
+  std::vector<int> rp, ci, cp, ri, csr2csc, csc2csr, csr_diag, csc_diag;
+  // Reference-order level sets (exported for parity)
+  pvglu::LevelSet lu, l, u, lt;
+  std::vector<int> max_per_level;
+
+  // Compiled gather schedule ------------------------------------------------
+  // refactor: for CSC slot q, ops [op_ptr[q], op_ptr[q+1]) are (L slot, U slot)
+  // pairs in ascending elimination column.
+  std::vector<int> op_ptr, op_l, op_u;
+  // forward substitution, row i: L(i,j) slots and columns j ascending
+  std::vector<int> lrow_ptr, lrow_slot, lrow_col;
+  // backward substitution, row i: U(i,j) slots and columns j descending
+  std::vector<int> urow_ptr, urow_slot, urow_col;
+  std::vector<int> diag_slot;
+  bool fuse_lsolve = false;
+  int ncomp = 0;
+  std::vector<int> comp_of; // component id per column
+  pvglu::KernelLevels k_lu, k_l, k_u, k_lt;
+  uint64_t hash = 0;
+
+  // schedule choice
+  int sched_variant = PVGLU_SCHED_AUTO;
+  int sched_width = 0;
+  int tuned_batch = -1;
+
+  pvglu::DevicePlan *dev = nullptr;
+};
+
+namespace pvglu {
+// Build the kernel views (component-contiguous chains) of a level set.
+void build_kernel_levels(const LevelSet &ls, const std::vector<int> &comp_of,
+                         int ncomp, KernelLevels &out);
+// Device teardown hook (defined in the HIP unit).
+void device_plan_destroy(pvglu_plan *plan);
+} // namespace pvglu

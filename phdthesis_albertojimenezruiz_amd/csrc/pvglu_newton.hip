@@ -1,0 +1,751 @@
+// pvglu_newton.hip -- Newton drivers of the config PV plants.
+//
+// Drop-in driver (pvglu_pv_newton_dropin): the reference's host call sequence
+// (PV/000_PVPP_simulator.cpp:163-235) -- host assembly on the map-based
+// matrix, symmetric + row permutation, CSC value fill, then the GPU refactor,
+// GPU solve and the 011 damped update on the GPU -- one instance at a time.
+//
+// Batched driver (pvglu_pv_newton_batched): B scenarios of one plant share
+// the pattern; every Newton iteration is one replay of a hipGraph:
+//   k_eval (module I(V), G = -dI/dV, I_eq per element and instance)
+//   -> k_stamp_a / k_stamp_b (gather of the companion stamps into the
+//      permuted CSC values and right-hand side, in the host assembly's order)
+//   -> refactor + solve (pvglu_refactor_solve)
+//   -> 011 update (pvglu_newton_update).
+// The adjoint (pvglu_pv_gradients) reuses the factors for A^T mu = dP/dx.
+#include <hip/hip_runtime.h>
+
+#include "pvglu_module.h"
+#include "pvglu_pv.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+using namespace pvglu;
+
+#define HIP_TRY(expr)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail(PVGLU_E_HIP, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                         \
+  } while (0)
+
+static const double kLadder[] = {100, 10, 1, 0.9, 0.8, 0.5, 0.4, 0.3, 0.2, 0.1,
+                                 0.05, -1}; // PV/000_PVPP_simulator.cpp:101-102
+
+namespace pvglu {
+
+struct DropinDev {
+  double *a = nullptr, *lu = nullptr, *b = nullptr, *x = nullptr, *v = nullptr,
+         *dsum = nullptr, *dmax = nullptr;
+  int *pos = nullptr, *neg = nullptr;
+};
+
+void dropin_dev_destroy(DropinDev *d) {
+  if (!d)
+    return;
+  for (void *p : {(void *)d->a, (void *)d->lu, (void *)d->b, (void *)d->x,
+                  (void *)d->v, (void *)d->dsum, (void *)d->dmax,
+                  (void *)d->pos, (void *)d->neg})
+    (void)hipFree(p);
+  delete d;
+}
+
+struct NewtonDev {
+  int B = 0;
+  // pattern-side (shared by instances)
+  double *base_a = nullptr, *base_b = nullptr, *rsh = nullptr,
+         *cab_g = nullptr;
+  int *a_ptr = nullptr, *a_mod = nullptr, *a_sgn = nullptr, *b_ptr = nullptr,
+      *b_mod = nullptr, *b_sgn = nullptr, *pos = nullptr, *neg = nullptr,
+      *rpos = nullptr, *rneg = nullptr, *cab_ra = nullptr, *cab_rb = nullptr,
+      *cab_ca = nullptr, *cab_cb = nullptr, *bus_c = nullptr;
+  // instance-side
+  double *G = nullptr, *v = nullptr, *geq = nullptr, *ieq = nullptr,
+         *A = nullptr, *LU = nullptr, *b = nullptr, *x = nullptr,
+         *dsum = nullptr, *dmax = nullptr, *gv = nullptr, *gP = nullptr,
+         *mu = nullptr, *dG = nullptr, *dgc = nullptr;
+  hipStream_t s = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  std::vector<void *> allocs;
+  std::vector<void *> inst_allocs;
+};
+
+void newton_dev_destroy(NewtonDev *d) {
+  if (!d)
+    return;
+  if (d->exec)
+    (void)hipGraphExecDestroy(d->exec);
+  if (d->graph)
+    (void)hipGraphDestroy(d->graph);
+  if (d->s)
+    (void)hipStreamDestroy(d->s);
+  for (void *p : d->allocs)
+    (void)hipFree(p);
+  for (void *p : d->inst_allocs)
+    (void)hipFree(p);
+  delete d;
+}
+
+} // namespace pvglu
+
+// ----------------------------------------------------------------- kernels
+
+// Companion model of each module (PV/002b_sistema_electrico_base_matriz.cpp:183-203):
+// element current f(V) = -I(V) flows + -> -, G = f'(V), I_eq = f(V) - G V.
+__global__ void k_eval(int nm, int B, const double *__restrict__ Girr,
+                       const double *__restrict__ rsh,
+                       const double *__restrict__ v, double *__restrict__ geq,
+                       double *__restrict__ ieq) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)nm * B)
+    return;
+  const int e = (int)(gid / B);
+  const double V = v[gid];
+  const ModuleEval me = module_current(V, kIphPerIrradiance * Girr[gid], rsh[e]);
+  const double f = -me.I, g = -me.dIdV;
+  geq[gid] = g;
+  ieq[gid] = f - g * V;
+}
+
+// A[q] = linear part + sum over the modules stamping slot q, each adding
+// sign*G and, on homotopy stages, sign*g_virtual.
+__global__ void k_stamp_a(int nnz, int B, const double *__restrict__ base,
+                          const int *__restrict__ ptr,
+                          const int *__restrict__ mod,
+                          const int *__restrict__ sgn,
+                          const double *__restrict__ geq,
+                          const double *__restrict__ gvp, double *A) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)nnz * B)
+    return;
+  const int q = (int)(gid / B), inst = (int)(gid % B);
+  const double gv = *gvp;
+  double a = base[q];
+  for (int k = ptr[q]; k < ptr[q + 1]; ++k) {
+    const double s = (double)sgn[k];
+    a = a + s * geq[(size_t)mod[k] * B + inst];
+    if (gv > 0)
+      a = a + s * gv;
+  }
+  A[gid] = a;
+}
+
+// b[r] = linear sources - sum over modules of sign * I_eq
+__global__ void k_stamp_b(int n, int B, const double *__restrict__ base,
+                          const int *__restrict__ ptr,
+                          const int *__restrict__ mod,
+                          const int *__restrict__ sgn,
+                          const double *__restrict__ ieq, double *b) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)n * B)
+    return;
+  const int r = (int)(gid / B), inst = (int)(gid % B);
+  double v = base[r];
+  for (int k = ptr[r]; k < ptr[r + 1]; ++k)
+    v = v - (double)sgn[k] * ieq[(size_t)mod[k] * B + inst];
+  b[gid] = v;
+}
+
+// dP/dx on the bus unknowns (permuted): P = sum_bus V G_N (V - 600);
+// zero elsewhere (k_zero first, then k_power_grad_bus)
+__global__ void k_zero(long count, double *p) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid < count)
+    p[gid] = 0.0;
+}
+
+__global__ void k_power_grad_bus(int nbus, int B, const int *__restrict__ bus_c,
+                                 const double *__restrict__ x, double *gP) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)nbus * B)
+    return;
+  const int k = (int)(gid / B), inst = (int)(gid % B);
+  const size_t idx = (size_t)bus_c[k] * B + inst;
+  gP[idx] = kNortonG * (2.0 * x[idx] - kNortonV);
+}
+
+// dP/dG_m = 9e-3 dI/dIph (mu[+] - mu[-]); mu in row-permuted order
+__global__ void k_grad_modules(int nm, int B, const double *__restrict__ Girr,
+                               const double *__restrict__ rsh,
+                               const double *__restrict__ v,
+                               const int *__restrict__ rpos,
+                               const int *__restrict__ rneg,
+                               const double *__restrict__ mu, double *dG) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)nm * B)
+    return;
+  const int e = (int)(gid / B), inst = (int)(gid % B);
+  const ModuleEval me =
+      module_current(v[gid], kIphPerIrradiance * Girr[gid], rsh[e]);
+  const double lp = rpos[e] >= 0 ? mu[(size_t)rpos[e] * B + inst] : 0.0;
+  const double ln = rneg[e] >= 0 ? mu[(size_t)rneg[e] * B + inst] : 0.0;
+  dG[gid] = kIphPerIrradiance * me.dIdIph * (lp - ln);
+}
+
+// dP/dg_c = -(mu[a] - mu[b]) (V_a - V_b)
+__global__ void k_grad_cables(int nc, int B, const int *__restrict__ ra,
+                              const int *__restrict__ rb,
+                              const int *__restrict__ ca,
+                              const int *__restrict__ cb,
+                              const double *__restrict__ x,
+                              const double *__restrict__ mu, double *dg) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= (long)nc * B)
+    return;
+  const int c = (int)(gid / B), inst = (int)(gid % B);
+  const double dv = x[(size_t)ca[c] * B + inst] - x[(size_t)cb[c] * B + inst];
+  const double dl = mu[(size_t)ra[c] * B + inst] - mu[(size_t)rb[c] * B + inst];
+  dg[gid] = -dl * dv;
+}
+
+// ------------------------------------------------------------ host helpers
+
+namespace {
+
+template <typename T>
+int dupload(std::vector<void *> &allocs, T **dst, const std::vector<T> &src) {
+  void *p = nullptr;
+  HIP_TRY(hipMalloc(&p, sizeof(T) * std::max<size_t>(src.size(), 1)));
+  allocs.push_back(p);
+  if (!src.empty())
+    HIP_TRY(hipMemcpy(p, src.data(), sizeof(T) * src.size(),
+                      hipMemcpyHostToDevice));
+  *dst = (T *)p;
+  return PVGLU_OK;
+}
+
+template <typename T>
+int dalloc(std::vector<void *> &allocs, T **dst, size_t count) {
+  void *p = nullptr;
+  HIP_TRY(hipMalloc(&p, sizeof(T) * std::max<size_t>(count, 1)));
+  allocs.push_back(p);
+  *dst = (T *)p;
+  return PVGLU_OK;
+}
+
+// slot of entry (r, c) in the plan (CSR search, then CSR->CSC map)
+int slot_of(const pvglu_plan *P, int r, int c) {
+  const int *b = P->ci.data() + P->rp[r], *e = P->ci.data() + P->rp[r + 1];
+  const int *it = std::lower_bound(b, e, c);
+  if (it == e || *it != c)
+    return -1;
+  return P->csr2csc[(int)(it - P->ci.data())];
+}
+
+// rowmap: original row -> permuted row; colmap: unknown -> permuted unknown
+void perm_maps(const pvglu_pv *pv, std::vector<int> &rowmap,
+               std::vector<int> &colmap) {
+  rowmap.resize(pv->n);
+  colmap.resize(pv->n);
+  for (int i = 0; i < pv->n; i++) {
+    colmap[i] = pv->Pinv[i];
+    rowmap[i] = pv->Prow[pv->Pinv[i]];
+  }
+}
+
+int newton_setup(pvglu_pv *pv) {
+  if (pv->nd)
+    return PVGLU_OK;
+  int st = pv_prepare(pv);
+  if (st)
+    return st;
+  const pvglu_plan *P = pv->plan;
+  const int n = pv->n, nnz = P->nnz, nm = pv->nmod;
+  std::vector<int> rowmap, colmap;
+  perm_maps(pv, rowmap, colmap);
+  // linear part in permuted CSC slots / rows
+  std::vector<double> base_a(nnz, 0.0), base_b(n, 0.0);
+  for (int r = 0; r < n; r++)
+    for (auto &e : pv->lin_A.rows[r]) {
+      const int q = slot_of(P, rowmap[r], colmap[e.first]);
+      if (q < 0)
+        return fail(PVGLU_E_ARG, "newton_setup: linear stamp outside pattern");
+      base_a[q] = e.second;
+    }
+  for (int r = 0; r < n; r++)
+    base_b[rowmap[r]] = pv->lin_b[r];
+  // module contributions per slot and per row, in element order
+  std::vector<std::vector<std::pair<int, int>>> as(nnz), bs(n);
+  for (int e = 0; e < nm; e++) {
+    const int p = pv->mod_pos[e], m = pv->mod_neg[e];
+    auto add = [&](int r, int c, int s) {
+      const int q = slot_of(P, rowmap[r], colmap[c]);
+      as[q].push_back({e, s});
+    };
+    if (p >= 0)
+      add(p, p, 1);
+    if (m >= 0)
+      add(m, m, 1);
+    if (p >= 0 && m >= 0) {
+      add(p, m, -1);
+      add(m, p, -1);
+    }
+    if (p >= 0)
+      bs[rowmap[p]].push_back({e, 1});
+    if (m >= 0)
+      bs[rowmap[m]].push_back({e, -1});
+  }
+  auto flat = [](const std::vector<std::vector<std::pair<int, int>>> &l,
+                 std::vector<int> &ptr, std::vector<int> &mod,
+                 std::vector<int> &sgn) {
+    ptr.assign(1, 0);
+    for (auto &v : l) {
+      for (auto &x : v) {
+        mod.push_back(x.first);
+        sgn.push_back(x.second);
+      }
+      ptr.push_back((int)mod.size());
+    }
+  };
+  std::vector<int> a_ptr, a_mod, a_sgn, b_ptr, b_mod, b_sgn;
+  flat(as, a_ptr, a_mod, a_sgn);
+  flat(bs, b_ptr, b_mod, b_sgn);
+  std::vector<int> pos(nm), neg(nm), rpos(nm), rneg(nm);
+  for (int e = 0; e < nm; e++) {
+    pos[e] = pv->mod_pos[e] >= 0 ? colmap[pv->mod_pos[e]] : -1;
+    neg[e] = pv->mod_neg[e] >= 0 ? colmap[pv->mod_neg[e]] : -1;
+    rpos[e] = pv->mod_pos[e] >= 0 ? rowmap[pv->mod_pos[e]] : -1;
+    rneg[e] = pv->mod_neg[e] >= 0 ? rowmap[pv->mod_neg[e]] : -1;
+  }
+  std::vector<int> cra(pv->ncab), crb(pv->ncab), cca(pv->ncab), ccb(pv->ncab),
+      busc(pv->I);
+  for (int c = 0; c < pv->ncab; c++) {
+    cra[c] = rowmap[pv->cab_a[c]];
+    crb[c] = rowmap[pv->cab_b[c]];
+    cca[c] = colmap[pv->cab_a[c]];
+    ccb[c] = colmap[pv->cab_b[c]];
+  }
+  for (int k = 0; k < pv->I; k++)
+    busc[k] = colmap[pv->bus[k]];
+  auto *d = new NewtonDev();
+  pv->nd = d;
+  auto &A = d->allocs;
+  if ((st = dupload(A, &d->base_a, base_a)) || (st = dupload(A, &d->base_b, base_b)) ||
+      (st = dupload(A, &d->rsh, pv->mod_rsh)) || (st = dupload(A, &d->cab_g, pv->cab_g)) ||
+      (st = dupload(A, &d->a_ptr, a_ptr)) || (st = dupload(A, &d->a_mod, a_mod)) ||
+      (st = dupload(A, &d->a_sgn, a_sgn)) || (st = dupload(A, &d->b_ptr, b_ptr)) ||
+      (st = dupload(A, &d->b_mod, b_mod)) || (st = dupload(A, &d->b_sgn, b_sgn)) ||
+      (st = dupload(A, &d->pos, pos)) || (st = dupload(A, &d->neg, neg)) ||
+      (st = dupload(A, &d->rpos, rpos)) || (st = dupload(A, &d->rneg, rneg)) ||
+      (st = dupload(A, &d->cab_ra, cra)) || (st = dupload(A, &d->cab_rb, crb)) ||
+      (st = dupload(A, &d->cab_ca, cca)) || (st = dupload(A, &d->cab_cb, ccb)) ||
+      (st = dupload(A, &d->bus_c, busc)) || (st = dalloc(A, &d->gv, 1)))
+    return st;
+  HIP_TRY(hipStreamCreate(&d->s));
+  return PVGLU_OK;
+}
+
+int newton_alloc_instances(pvglu_pv *pv, int B) {
+  NewtonDev *d = pv->nd;
+  if (d->B == B)
+    return PVGLU_OK;
+  if (d->exec) {
+    (void)hipGraphExecDestroy(d->exec);
+    d->exec = nullptr;
+  }
+  if (d->graph) {
+    (void)hipGraphDestroy(d->graph);
+    d->graph = nullptr;
+  }
+  for (void *p : d->inst_allocs)
+    (void)hipFree(p);
+  d->inst_allocs.clear();
+  d->B = 0;
+  const size_t nm = pv->nmod, n = pv->n, nnz = pv->plan->nnz, nc = pv->ncab;
+  auto &I = d->inst_allocs;
+  int st;
+  if ((st = dalloc(I, &d->G, nm * B)) || (st = dalloc(I, &d->v, nm * B)) ||
+      (st = dalloc(I, &d->geq, nm * B)) || (st = dalloc(I, &d->ieq, nm * B)) ||
+      (st = dalloc(I, &d->A, nnz * B)) || (st = dalloc(I, &d->LU, nnz * B)) ||
+      (st = dalloc(I, &d->b, n * B)) || (st = dalloc(I, &d->x, n * B)) ||
+      (st = dalloc(I, &d->dsum, (size_t)B)) || (st = dalloc(I, &d->dmax, (size_t)B)) ||
+      (st = dalloc(I, &d->gP, n * B)) || (st = dalloc(I, &d->mu, n * B)) ||
+      (st = dalloc(I, &d->dG, nm * B)) || (st = dalloc(I, &d->dgc, nc * B)))
+    return st;
+  d->B = B;
+  return PVGLU_OK;
+}
+
+inline dim3 grid_for(long count) { return dim3((unsigned)((count + 255) / 256)); }
+
+// eval + stamps on stream s
+int launch_stamp(pvglu_pv *pv, int B) {
+  NewtonDev *d = pv->nd;
+  const int nm = pv->nmod, n = pv->n, nnz = pv->plan->nnz;
+  hipLaunchKernelGGL(k_eval, grid_for((long)nm * B), dim3(256), 0, d->s, nm, B,
+                     d->G, d->rsh, d->v, d->geq, d->ieq);
+  hipLaunchKernelGGL(k_stamp_a, grid_for((long)nnz * B), dim3(256), 0, d->s,
+                     nnz, B, d->base_a, d->a_ptr, d->a_mod, d->a_sgn, d->geq,
+                     d->gv, d->A);
+  hipLaunchKernelGGL(k_stamp_b, grid_for((long)n * B), dim3(256), 0, d->s, n,
+                     B, d->base_b, d->b_ptr, d->b_mod, d->b_sgn, d->ieq, d->b);
+  HIP_TRY(hipGetLastError());
+  return PVGLU_OK;
+}
+
+int launch_iteration(pvglu_pv *pv, int B) {
+  NewtonDev *d = pv->nd;
+  int st = launch_stamp(pv, B);
+  if (st)
+    return st;
+  if ((st = pvglu_refactor_solve(pv->plan, d->A, d->LU, d->b, d->x, B, d->s)))
+    return st;
+  return pvglu_newton_update(pv->plan, pv->nmod, B, d->pos, d->neg, d->x, d->v,
+                             d->dsum, d->dmax, d->s);
+}
+
+int build_graph(pvglu_pv *pv, int B) {
+  NewtonDev *d = pv->nd;
+  if (d->exec)
+    return PVGLU_OK;
+  HIP_TRY(hipStreamBeginCapture(d->s, hipStreamCaptureModeThreadLocal));
+  int st = launch_iteration(pv, B);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(d->s, &g);
+  if (st) {
+    if (g)
+      (void)hipGraphDestroy(g);
+    return st;
+  }
+  if (e != hipSuccess)
+    return fail(PVGLU_E_HIP, "graph capture: %s", hipGetErrorString(e));
+  d->graph = g;
+  HIP_TRY(hipGraphInstantiate(&d->exec, g, nullptr, nullptr, 0));
+  return PVGLU_OK;
+}
+
+} // namespace
+
+// ------------------------------------------------------------- batched API
+
+static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
+                         double tol, int max_iters, int ladder,
+                         double *stats) {
+  int st = newton_setup(pv);
+  if (st)
+    return st;
+  if ((st = newton_alloc_instances(pv, B)))
+    return st;
+  NewtonDev *d = pv->nd;
+  const size_t nm = pv->nmod;
+  // pick the kernel schedule before capture (autotune allocates + syncs)
+  if (pv->plan->tuned_batch != B && pv->plan->sched_width == 0)
+    if ((st = pvglu_autotune(pv->plan, B, 3, nullptr)))
+      return st;
+  HIP_TRY(hipMemcpyAsync(d->G, g, sizeof(double) * nm * B,
+                         hipMemcpyHostToDevice, d->s));
+  HIP_TRY(hipMemcpyAsync(d->v, v_module, sizeof(double) * nm * B,
+                         hipMemcpyHostToDevice, d->s));
+  if ((st = pvglu_status_reset(pv->plan, d->s)))
+    return st;
+  if ((st = build_graph(pv, B)))
+    return st;
+  std::vector<double> hsum(B), hmax(B);
+  const int nst = ladder ? (int)(sizeof kLadder / sizeof kLadder[0]) : 1;
+  const double *stages = ladder ? kLadder : kLadder + 11;
+  int iters = 0;
+  double last_max = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  for (int si = 0; si < nst; si++) {
+    const double gv = stages[si];
+    const bool last = si == nst - 1;
+    HIP_TRY(hipMemcpyAsync(d->gv, &gv, sizeof(double), hipMemcpyHostToDevice,
+                           d->s));
+    for (;;) {
+      if (iters >= max_iters)
+        return fail(PVGLU_E_NOCONV, "batched Newton: %d iterations", iters);
+      HIP_TRY(hipGraphLaunch(d->exec, d->s));
+      iters++;
+      HIP_TRY(hipMemcpyAsync(hsum.data(), d->dsum, sizeof(double) * B,
+                             hipMemcpyDeviceToHost, d->s));
+      HIP_TRY(hipMemcpyAsync(hmax.data(), d->dmax, sizeof(double) * B,
+                             hipMemcpyDeviceToHost, d->s));
+      HIP_TRY(hipStreamSynchronize(d->s));
+      double msum = 0, mmax = 0;
+      for (int k = 0; k < B; k++) {
+        msum = std::max(msum, hsum[k]);
+        mmax = std::max(mmax, hmax[k]);
+      }
+      last_max = mmax;
+      if (std::isnan(msum) || std::isnan(mmax))
+        return fail(PVGLU_E_NOCONV, "batched Newton: NaN update");
+      if (!last && msum < 1e-6 * pv->n)
+        break; // reference stage criterion (000:170-172)
+      if (last && mmax <= tol)
+        break;
+    }
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  int code = 0, col = 0, inst = 0;
+  if ((st = pvglu_status(pv->plan, &code, &col, &inst)))
+    return st;
+  if (code)
+    return fail(code, "batched Newton: status %d at column/element %d, instance %d",
+                code, col, inst);
+  HIP_TRY(hipMemcpy(v_module, d->v, sizeof(double) * nm * B,
+                    hipMemcpyDeviceToHost));
+  if (stats) {
+    const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    stats[0] = iters;
+    stats[1] = ms;
+    stats[2] = ms / std::max(iters, 1);
+    stats[3] = last_max;
+  }
+  return PVGLU_OK;
+}
+
+static int fetch_solution(pvglu_pv *pv, int B, double *x_out,
+                          double *power_out) {
+  NewtonDev *d = pv->nd;
+  const int n = pv->n;
+  std::vector<double> xp((size_t)n * B);
+  HIP_TRY(hipMemcpy(xp.data(), d->x, sizeof(double) * xp.size(),
+                    hipMemcpyDeviceToHost));
+  if (x_out)
+    for (int i = 0; i < n; i++)
+      memcpy(x_out + (size_t)i * B, xp.data() + (size_t)pv->Pinv[i] * B,
+             sizeof(double) * B);
+  if (power_out)
+    for (int k = 0; k < B; k++) {
+      double p = 0;
+      for (int bidx : pv->bus) {
+        const double V = xp[(size_t)pv->Pinv[bidx] * B + k];
+        p += V * (kNortonG * (V - kNortonV));
+      }
+      power_out[k] = p;
+    }
+  return PVGLU_OK;
+}
+
+extern "C" int pvglu_pv_newton_batched(pvglu_pv *pv, int batch, const double *g,
+                                       double *v_module, double *x_out,
+                                       double *power_out, double tol,
+                                       int max_iters, int ladder,
+                                       double *stats) {
+  clear_error();
+  if (!pv || batch <= 0 || !g || !v_module)
+    return fail(PVGLU_E_ARG, "newton_batched: bad arguments");
+  int st = batched_solve(pv, batch, g, v_module, tol,
+                         max_iters > 0 ? max_iters : 5000, ladder, stats);
+  if (st)
+    return st;
+  return fetch_solution(pv, batch, x_out, power_out);
+}
+
+extern "C" int pvglu_pv_gradients(pvglu_pv *pv, int batch, const double *g,
+                                  double *v_module, double *power_out,
+                                  double *dP_dG, double *dP_dgc, double tol,
+                                  int max_iters) {
+  clear_error();
+  if (!pv || batch <= 0 || !g || !v_module)
+    return fail(PVGLU_E_ARG, "gradients: bad arguments");
+  int st = batched_solve(pv, batch, g, v_module, tol,
+                         max_iters > 0 ? max_iters : 5000, 0, nullptr);
+  if (st)
+    return st;
+  NewtonDev *d = pv->nd;
+  const int B = batch, n = pv->n, nm = pv->nmod, nc = pv->ncab;
+  // Jacobian at the converged element voltages: stamp + refactor (the
+  // companion matrix at V_e = V_branch(x) is dF/dx), then the last solve's x
+  if ((st = launch_stamp(pv, B)))
+    return st;
+  if ((st = pvglu_refactor_solve(pv->plan, d->A, d->LU, d->b, d->x, B, d->s)))
+    return st;
+  hipLaunchKernelGGL(k_zero, grid_for((long)n * B), dim3(256), 0, d->s,
+                     (long)n * B, d->gP);
+  hipLaunchKernelGGL(k_power_grad_bus, grid_for((long)pv->I * B), dim3(256), 0,
+                     d->s, pv->I, B, d->bus_c, d->x, d->gP);
+  if ((st = pvglu_solve_transpose(pv->plan, d->LU, d->gP, d->mu, B, d->s)))
+    return st;
+  hipLaunchKernelGGL(k_grad_modules, grid_for((long)nm * B), dim3(256), 0,
+                     d->s, nm, B, d->G, d->rsh, d->v, d->rpos, d->rneg, d->mu,
+                     d->dG);
+  hipLaunchKernelGGL(k_grad_cables, grid_for((long)nc * B), dim3(256), 0, d->s,
+                     nc, B, d->cab_ra, d->cab_rb, d->cab_ca, d->cab_cb, d->x,
+                     d->mu, d->dgc);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(d->s));
+  int code = 0, col = 0, inst = 0;
+  if ((st = pvglu_status(pv->plan, &code, &col, &inst)))
+    return st;
+  if (code)
+    return fail(code, "gradients: status %d", code);
+  if (dP_dG)
+    HIP_TRY(hipMemcpy(dP_dG, d->dG, sizeof(double) * nm * B,
+                      hipMemcpyDeviceToHost));
+  if (dP_dgc)
+    HIP_TRY(hipMemcpy(dP_dgc, d->dgc, sizeof(double) * nc * B,
+                      hipMemcpyDeviceToHost));
+  return fetch_solution(pv, B, nullptr, power_out);
+}
+
+// ------------------------------------------------------------- drop-in API
+
+extern "C" int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g,
+                                      double *v_module, double *x_out,
+                                      double tol, int ladder, int fixed_iters,
+                                      double *stats) {
+  clear_error();
+  if (!pv || !g || !v_module)
+    return fail(PVGLU_E_ARG, "newton_dropin: bad arguments");
+  int st = pv_prepare(pv);
+  if (st)
+    return st;
+  pvglu_plan *P = pv->plan;
+  const int n = pv->n, nnz = P->nnz, nm = pv->nmod;
+  if (!pv->dd) {
+    auto *d = new DropinDev();
+    pv->dd = d;
+    HIP_TRY(hipMalloc(&d->a, sizeof(double) * nnz));
+    HIP_TRY(hipMalloc(&d->lu, sizeof(double) * nnz));
+    HIP_TRY(hipMalloc(&d->b, sizeof(double) * n));
+    HIP_TRY(hipMalloc(&d->x, sizeof(double) * n));
+    HIP_TRY(hipMalloc(&d->v, sizeof(double) * nm));
+    HIP_TRY(hipMalloc(&d->dsum, sizeof(double)));
+    HIP_TRY(hipMalloc(&d->dmax, sizeof(double)));
+    HIP_TRY(hipMalloc(&d->pos, sizeof(int) * nm));
+    HIP_TRY(hipMalloc(&d->neg, sizeof(int) * nm));
+    std::vector<int> pos(nm), neg(nm);
+    for (int e = 0; e < nm; e++) {
+      pos[e] = pv->mod_pos[e] >= 0 ? pv->Pinv[pv->mod_pos[e]] : -1;
+      neg[e] = pv->mod_neg[e] >= 0 ? pv->Pinv[pv->mod_neg[e]] : -1;
+    }
+    HIP_TRY(hipMemcpy(d->pos, pos.data(), sizeof(int) * nm, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(d->neg, neg.data(), sizeof(int) * nm, hipMemcpyHostToDevice));
+  }
+  DropinDev *d = pv->dd;
+  for (int e = 0; e < nm; e++)
+    pv->mod_iph[e] = kIphPerIrradiance * g[e];
+  pv->circuit.v_iterate.assign(v_module, v_module + nm);
+  HIP_TRY(hipMemcpy(d->v, v_module, sizeof(double) * nm, hipMemcpyHostToDevice));
+  if ((st = pvglu_status_reset(P, nullptr)))
+    return st;
+  std::vector<double> vals(nnz), bvec, tmp(n);
+  double t_host = 0, t_dev = 0;
+  auto T0 = std::chrono::steady_clock::now();
+  int it = 0;
+  const int nst = (ladder && fixed_iters <= 0) ? 12 : 1;
+  const double *stages = (ladder && fixed_iters <= 0) ? kLadder : kLadder + 11;
+  for (int si = 0; si < nst; si++) {
+    const bool last = si == nst - 1;
+    long double diff = 1e10;
+    int stage_it = 0;
+    for (;;) {
+      if (fixed_iters > 0) {
+        if (stage_it >= fixed_iters)
+          break;
+      } else if (!last && diff < 1e-6L * n) {
+        break;
+      }
+      if (++it > 5000)
+        return fail(PVGLU_E_NOCONV, "Did not converge"); // 000:179-181
+      stage_it++;
+      auto h0 = std::chrono::steady_clock::now();
+      // create_matrix_and_vector + permutations + CSC value fill (000:185-210)
+      SparseMap A;
+      pv->circuit.create_matrix_and_vector(stages[si], A, bvec);
+      A = A.interchange_row_columns(pv->Pinv.data());
+      A = A.exchange_rows(pv->Prow.data());
+      for (int i = 0; i < n; i++)
+        tmp[pv->Pinv[i]] = bvec[i];
+      for (int i = 0; i < n; i++)
+        bvec[pv->Prow[i]] = tmp[i];
+      for (int c = 0; c < n; c++)
+        for (int q = P->cp[c]; q < P->cp[c + 1]; q++)
+          vals[q] = A.D(P->ri[q], c); // fill_values_CSC_only (010:142-155)
+      auto h1 = std::chrono::steady_clock::now();
+      HIP_TRY(hipMemcpy(d->a, vals.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(d->b, bvec.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+      if ((st = pvglu_refactor(P, d->a, d->lu, 1, nullptr)) ||
+          (st = pvglu_solve(P, d->lu, d->b, d->x, 1, nullptr)) ||
+          (st = pvglu_newton_update(P, nm, 1, d->pos, d->neg, d->x, d->v, d->dsum,
+                                    d->dmax, nullptr)))
+        return st;
+      double dsum = 0, dmax = 0;
+      HIP_TRY(hipMemcpy(pv->circuit.v_iterate.data(), d->v, sizeof(double) * nm,
+                        hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(&dsum, d->dsum, sizeof(double), hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(&dmax, d->dmax, sizeof(double), hipMemcpyDeviceToHost));
+      auto h2 = std::chrono::steady_clock::now();
+      t_host += std::chrono::duration<double, std::milli>(h1 - h0).count();
+      t_dev += std::chrono::duration<double, std::milli>(h2 - h1).count();
+      diff = dsum;
+      if (std::isnan(dsum))
+        return fail(PVGLU_E_NOCONV, "drop-in Newton: NaN update");
+      if (fixed_iters <= 0 && last && dmax <= tol)
+        break;
+    }
+  }
+  int code = 0, col = 0, inst = 0;
+  if ((st = pvglu_status(P, &code, &col, &inst)))
+    return st;
+  if (code)
+    return fail(code, "drop-in Newton: status %d at %d", code, col);
+  memcpy(v_module, pv->circuit.v_iterate.data(), sizeof(double) * nm);
+  if (x_out) {
+    HIP_TRY(hipMemcpy(tmp.data(), d->x, sizeof(double) * n, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; i++)
+      x_out[i] = tmp[pv->Pinv[i]]; // apply_permutation(x, P) (000:233)
+  }
+  auto T1 = std::chrono::steady_clock::now();
+  if (stats) {
+    stats[0] = it;
+    stats[1] = t_host;
+    stats[2] = t_dev;
+    stats[3] = std::chrono::duration<double, std::milli>(T1 - T0).count();
+  }
+  return PVGLU_OK;
+}
+
+// ------------------------------------------------------ stamping entry points
+
+extern "C" pvglu_plan *pvglu_pv_plan(pvglu_pv *pv) {
+  clear_error();
+  if (!pv) {
+    fail(PVGLU_E_ARG, "pv_plan: null");
+    return nullptr;
+  }
+  if (pv_prepare(pv))
+    return nullptr;
+  return pv->plan;
+}
+
+extern "C" int pvglu_pv_stamp(pvglu_pv *pv, int batch, const double *g,
+                              const double *v, double g_virtual, double *a,
+                              double *b, void *stream) {
+  clear_error();
+  if (!pv || batch <= 0 || !g || !v || !a || !b)
+    return fail(PVGLU_E_ARG, "pv_stamp: bad arguments");
+  int st = newton_setup(pv);
+  if (st)
+    return st;
+  NewtonDev *d = pv->nd;
+  const int nm = pv->nmod, n = pv->n, nnz = pv->plan->nnz, B = batch;
+  hipStream_t s = (hipStream_t)stream;
+  // element companion values in caller-independent scratch sized for B
+  double *geq = nullptr, *ieq = nullptr;
+  if (d->B >= B) {
+    geq = d->geq;
+    ieq = d->ieq;
+  } else {
+    if ((st = newton_alloc_instances(pv, B)))
+      return st;
+    geq = d->geq;
+    ieq = d->ieq;
+  }
+  HIP_TRY(hipMemcpyAsync(d->gv, &g_virtual, sizeof(double), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_eval, grid_for((long)nm * B), dim3(256), 0, s, nm, B, g,
+                     d->rsh, v, geq, ieq);
+  hipLaunchKernelGGL(k_stamp_a, grid_for((long)nnz * B), dim3(256), 0, s, nnz,
+                     B, d->base_a, d->a_ptr, d->a_mod, d->a_sgn, geq, d->gv, a);
+  hipLaunchKernelGGL(k_stamp_b, grid_for((long)n * B), dim3(256), 0, s, n, B,
+                     d->base_b, d->b_ptr, d->b_mod, d->b_sgn, ieq, b);
+  HIP_TRY(hipGetLastError());
+  return PVGLU_OK;
+}

@@ -83,3 +83,47 @@ def dd_values(rng, n, rp, ci):
         if d >= 0:
             vals[d] = (s + 1.0) * (1 if rng.random() < 0.5 else -1)
     return vals
+
+
+def plant_values(rng, I, S, M, B=None):
+    """MNA values on ``plant_pattern(I, S, M)``: per module a conductance
+    (diode companion, 1e-3..5 S), per cable 5..50 S, per bus a 1e4 S Norton
+    conductance, and a right-hand side.  Returns (n, rp, ci, vals, b) with
+    CSR values; with ``B`` the values are [nnz][B] and b is [n][B]
+    (independent draws per instance)."""
+    n, rows = plant_pattern(I, S, M)
+    rp, ci = csr_from_sets(rows)
+    where = {}
+    for i in range(n):
+        for p in range(rp[i], rp[i + 1]):
+            where[(i, int(ci[p]))] = p
+    nb = 1 if B is None else B
+    nm = I * S * M
+    node = np.arange(nm) + np.arange(nm) // (S * M)
+    first = (np.arange(nm) % M) == 0
+    prev = np.where(first, -1, node - 1)
+    comb = np.array([inv * (S * M + 1) + s * M + M - 1 for inv in range(I) for s in range(S)])
+    bus = np.array([inv * (S * M + 1) + S * M for inv in range(I) for s in range(S)])
+    buses = np.array([inv * (S * M + 1) + S * M for inv in range(I)])
+    gm = np.exp(rng.uniform(np.log(1e-3), np.log(5.0), (nm, nb)))
+    gc = np.exp(rng.uniform(np.log(5.0), np.log(50.0), (len(comb), nb)))
+    ieq = rng.uniform(-9.0, 9.0, (nm, nb))
+    vals = np.zeros((len(ci), nb))
+    bvec = np.zeros((n, nb))
+    idx = lambda a, b: np.array([where[(int(x), int(y))] for x, y in zip(a, b)], dtype=np.int64)
+    np.add.at(vals, idx(node, node), gm)
+    has = ~first
+    np.add.at(vals, idx(prev[has], prev[has]), gm[has])
+    np.add.at(vals, idx(node[has], prev[has]), -gm[has])
+    np.add.at(vals, idx(prev[has], node[has]), -gm[has])
+    np.add.at(vals, idx(comb, comb), gc)
+    np.add.at(vals, idx(bus, bus), gc)
+    np.add.at(vals, idx(comb, bus), -gc)
+    np.add.at(vals, idx(bus, comb), -gc)
+    np.add.at(vals, idx(buses, buses), 1e4)
+    np.add.at(bvec, node, ieq)
+    np.add.at(bvec, prev[has], -ieq[has])
+    bvec[buses] += 1e4 * 600.0
+    if B is None:
+        return n, rp, ci, vals[:, 0], bvec[:, 0]
+    return n, rp, ci, vals, bvec
