@@ -198,11 +198,13 @@ int pvglu_multi_solve_transpose(pvglu_multi *m, const double *const *lu,
  * pos/neg: permuted row of each element's terminals (-1 = ground);
  * v_elem: element voltages [elem][instance], updated in place.
  * delta_sum / delta_max: per-instance sum and max of |delta| (device,
- * batch entries each).  Damping below 1e-13 records PVGLU_E_NOCONV in the
- * status word of `plan`. */
+ * batch entries each).  active (device, batch ints, may be NULL): instances
+ * with 0 are left untouched (converged).  Damping below 1e-13 records
+ * PVGLU_E_NOCONV in the status word of `plan`. */
 int pvglu_newton_update(pvglu_plan *plan, int ne, int batch, const int *pos,
                         const int *neg, const double *x, double *v_elem,
-                        double *delta_sum, double *delta_max, void *stream);
+                        double *delta_sum, double *delta_max,
+                        const int *active, void *stream);
 
 /* ---------------------------------------------------- PV plant harness */
 
@@ -266,7 +268,8 @@ int pvglu_reorder(int n, const int *rowptr, const int *colidx, int *P,
  * iteration host assembly, permutation, CSC value fill, GPU refactor, GPU
  * solve, un-permutation and the 011 update.  g: irradiance per module;
  * v_module: initial/final module voltages (in/out); x_out: node voltages (n).
- * tol: stop when max |delta| <= tol on the last homotopy stage;
+ * tol: stop when max |delta| <= tol on the last homotopy stage (or after
+ * 20 iterations with max |delta| <= 1e-8 V: the FP64 noise floor);
  * ladder: 1 = the reference's conductance ladder, 0 = last stage only.
  * fixed_iters > 0 runs exactly that many iterations of the last stage.
  * stats[0..3]: iterations, host assembly ms, device ms, total ms. */
@@ -284,6 +287,12 @@ int pvglu_pv_newton_batched(pvglu_pv *pv, int batch, const double *g,
                             double *v_module, double *x_out, double *power_out,
                             double tol, int max_iters, int ladder,
                             double *stats);
+
+/* Per-instance iteration counts and final max |delta| (V) of the last
+ * batched solve.  An instance whose final max |delta| exceeds tol stopped at
+ * the FP64 noise floor (20 iterations with max |delta| <= 1e-8 V). */
+int pvglu_pv_newton_info(const pvglu_pv *pv, int batch, int *iters,
+                         double *last_max_delta);
 
 /* Adjoint gradients of total power at a converged batched solution:
  * dP/dG per module [module][instance] and dP/dg per cable [cable][instance].

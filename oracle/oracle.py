@@ -346,6 +346,24 @@ class PVNewton:
             out[:, e] = module_current(V[e], iph[e], self.I0, self.A, self.RS, self.rsh[e])
         return out
 
+    def assemble(self, v, iph, gv):
+        """Permuted CSC values and right-hand side of the companion system."""
+        I, dIdV, _ = self.module(v, iph)
+        f, g = -I, -dIdV
+        ieq = f - g * v
+        vals = self.abase.copy()
+        np.add.at(vals, self.st_slot, self.st_sign * g[self.st_elem])
+        if gv > 0:
+            np.add.at(vals, self.st_slot, self.st_sign * gv)
+        b = self.bbase.copy()
+        pm = self.pos >= 0
+        np.add.at(b, self.pos[pm], -ieq[pm])
+        nm = self.neg >= 0
+        np.add.at(b, self.neg[nm], ieq[nm])
+        bp = np.zeros(self.n)
+        bp[self.rowmap] = b
+        return vals, bp
+
     def iterate(self, v, iph, gv):
         I, dIdV, _ = self.module(v, iph)
         f, g = -I, -dIdV
@@ -373,13 +391,18 @@ class PVNewton:
             raise RuntimeError("oracle Newton: damping below 1e-13")
         return x, v2, diff, mx, vals
 
+    NOISE, STALL = 1e-8, 20   # last-stage noise-floor rule (as the library)
+
     def solve(self, iph, v0, ladder=True, tol=1e-10, max_iters=5000):
+        """Returns (x, v, iterations); self.last_max holds the final max |delta|
+        (> tol when the iteration stopped at the FP64 noise floor)."""
         v = np.asarray(v0, float).copy()
         it = 0
         stages = LADDER if ladder else LADDER[-1:]
         x = None
         for k, gv in enumerate(stages):
             last = k == len(stages) - 1
+            stall = 0
             while True:
                 it += 1
                 if it > max_iters:
@@ -387,9 +410,14 @@ class PVNewton:
                 x, v, diff, mx, lu = self.iterate(v, iph, gv)
                 if not last and diff < 1e-6 * self.n:
                     break
-                if last and mx <= tol:
-                    break
+                if last:
+                    if mx <= tol:
+                        break
+                    stall = stall + 1 if mx <= self.NOISE else 0
+                    if stall >= self.STALL:
+                        break
         self.lu = lu
+        self.last_max = mx
         return x, v, it
 
     def power(self, x, buses):

@@ -79,7 +79,7 @@ def lib():
         "pvglu_status": (C.c_int, [_VP, _P_INT, _P_INT, _P_INT]),
         "pvglu_status_reset": (C.c_int, [_VP, _VP]),
         "pvglu_newton_update": (C.c_int, [_VP, C.c_int, C.c_int, _VP, _VP, _VP, _VP, _VP,
-                                          _VP, _VP]),
+                                          _VP, _VP, _VP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -132,9 +132,10 @@ class Plan:
         self.n = int(n)
 
     def close(self):
-        if getattr(self, "_h", None):
-            lib().pvglu_plan_destroy(self._h)
-            self._h = None
+        h = getattr(self, "_h", None)
+        self._h = None
+        if h and _lib is not None:
+            _lib.pvglu_plan_destroy(h)
 
     __del__ = close
 
@@ -195,10 +196,11 @@ class Plan:
     def status_reset(self, stream=None):
         _check(lib().pvglu_status_reset(self.handle, _stream(stream)))
 
-    def newton_update(self, ne, batch, pos, neg, x, v, dsum, dmax, stream=None):
+    def newton_update(self, ne, batch, pos, neg, x, v, dsum, dmax, active=None,
+                      stream=None):
         _check(lib().pvglu_newton_update(self.handle, int(ne), int(batch), _ptr(pos), _ptr(neg),
                                          _ptr(x), _ptr(v), _ptr(dsum), _ptr(dmax),
-                                         _stream(stream)))
+                                         _ptr(active), _stream(stream)))
 
 
 def obtain_fillin(n, rowptr, colidx, stream=None):
@@ -244,6 +246,7 @@ _PV_SIG = {
     "pvglu_pv_gradients": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, _VP, _VP, C.c_double,
                                      C.c_int]),
     "pvglu_pv_plan": (_VP, [_VP]),
+    "pvglu_pv_newton_info": (C.c_int, [_VP, C.c_int, _VP, _VP]),
     "pvglu_pv_stamp": (C.c_int, [_VP, C.c_int, _VP, _VP, C.c_double, _VP, _VP, _VP]),
 }
 
@@ -303,9 +306,10 @@ class PV:
         self.nnz_a = L.pvglu_pv_info(h, 3)
 
     def close(self):
-        if getattr(self, "_h", None):
-            _pvlib().pvglu_pv_destroy(self._h)
-            self._h = None
+        h = getattr(self, "_h", None)
+        self._h = None
+        if h and _lib is not None:
+            _lib.pvglu_pv_destroy(h)
 
     __del__ = close
 
@@ -378,18 +382,31 @@ class PV:
                                                int(bool(ladder)), int(fixed_iters), _dp(stats)))
         return x, v, stats
 
-    def newton_batched(self, g, v=None, tol=1e-10, max_iters=5000, ladder=True):
-        """g, v: [module][instance].  Returns (x [node][inst], v, power, stats)."""
+    def newton_batched(self, g, v=None, tol=1e-10, max_iters=5000, ladder=True,
+                       raise_noconv=True):
+        """g, v: [module][instance].  Returns (x [node][inst], v, power, stats).
+        With raise_noconv=False an iteration cap returns (None, v, None, None)
+        with the iterate reached."""
         g = np.ascontiguousarray(g, np.float64)
         B = g.shape[1]
         v = self.v0(B) if v is None else np.ascontiguousarray(v, np.float64).copy()
         x = np.zeros((self.n, B))
         p = np.zeros(B)
         stats = np.zeros(4)
-        _check(_pvlib().pvglu_pv_newton_batched(self._h, B, _dp(g), _dp(v), _dp(x), _dp(p),
+        code = _pvlib().pvglu_pv_newton_batched(self._h, B, _dp(g), _dp(v), _dp(x), _dp(p),
                                                 float(tol), int(max_iters), int(bool(ladder)),
-                                                _dp(stats)))
+                                                _dp(stats))
+        if code == E_NOCONV and not raise_noconv:
+            return None, v, None, None
+        _check(code)
         return x, v, p, stats
+
+    def newton_info(self, batch):
+        """Per-instance iterations and final max |delta| of the last batched solve."""
+        it = np.zeros(batch, np.int32)
+        lm = np.zeros(batch)
+        _check(_pvlib().pvglu_pv_newton_info(self._h, int(batch), _dp(it), _dp(lm)))
+        return it, lm
 
     def gradients(self, g, v=None, tol=1e-12, max_iters=5000):
         """Adjoint dP/dG [module][inst] and dP/dg_cable [cable][inst]."""
@@ -402,3 +419,46 @@ class PV:
         _check(_pvlib().pvglu_pv_gradients(self._h, B, _dp(g), _dp(v), _dp(p), _dp(dG),
                                            _dp(dgc), float(tol), int(max_iters)))
         return p, dG, dgc, v
+
+
+class Multi:
+    """Mixed-pattern batch (``pvglu_multi_*``): groups of instances, each
+    group sharing one plan, refactored and solved in one launch."""
+
+    def __init__(self, plans, batches):
+        L = lib()
+        L.pvglu_multi_create.restype = C.c_int
+        L.pvglu_multi_create.argtypes = [C.c_int, C.POINTER(_VP), _P_INT, C.POINTER(_VP)]
+        L.pvglu_multi_destroy.argtypes = [_VP]
+        for name in ("pvglu_multi_refactor_solve", "pvglu_multi_solve_transpose"):
+            getattr(L, name).restype = C.c_int
+        L.pvglu_multi_refactor_solve.argtypes = [_VP, C.POINTER(_VP), C.POINTER(_VP),
+                                                 C.POINTER(_VP), C.POINTER(_VP), _VP]
+        L.pvglu_multi_solve_transpose.argtypes = [_VP, C.POINTER(_VP), C.POINTER(_VP),
+                                                  C.POINTER(_VP), _VP]
+        self._plans = list(plans)          # keep the owners alive
+        hs = (_VP * len(plans))(*[p.handle for p in plans])
+        bs = (C.c_int * len(plans))(*[int(b) for b in batches])
+        h = _VP()
+        _check(L.pvglu_multi_create(len(plans), hs, bs, C.byref(h)))
+        self._h = h
+        self.ngroups = len(plans)
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        self._h = None
+        if h and _lib is not None:
+            _lib.pvglu_multi_destroy(h)
+
+    __del__ = close
+
+    def _arr(self, ts):
+        return (_VP * self.ngroups)(*[_ptr(t) for t in ts])
+
+    def refactor_solve(self, a, lu, b, x, stream=None):
+        _check(lib().pvglu_multi_refactor_solve(self._h, self._arr(a), self._arr(lu),
+                                                self._arr(b), self._arr(x), _stream(stream)))
+
+    def solve_transpose(self, lu, g, y, stream=None):
+        _check(lib().pvglu_multi_solve_transpose(self._h, self._arr(lu), self._arr(g),
+                                                 self._arr(y), _stream(stream)))

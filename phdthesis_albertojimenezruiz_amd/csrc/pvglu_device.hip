@@ -55,6 +55,8 @@ struct DevicePlan {
       *lrow_slot = nullptr, *lrow_col = nullptr, *urow_ptr = nullptr,
       *urow_slot = nullptr, *urow_col = nullptr;
   DevLevels lu, l, u, lt;
+  int *lu_rec = nullptr, *lu_fast = nullptr, *u_rec = nullptr,
+      *u_fast = nullptr;
   int *status = nullptr; // [code, column, instance, pad]
   std::vector<void *> allocs;
 };
@@ -75,6 +77,10 @@ struct Pat {
   const int *__restrict__ urow_ptr;
   const int *__restrict__ urow_slot;
   const int *__restrict__ urow_col;
+  const int *__restrict__ lu_rec;
+  const int *__restrict__ lu_fast;
+  const int *__restrict__ u_rec;
+  const int *__restrict__ u_fast;
   int *status;
 };
 
@@ -90,7 +96,8 @@ struct Lev {
 static Pat pat_of(const DevicePlan *d) {
   return Pat{d->cp,       d->ri,        d->op_ptr,    d->op_l,     d->op_u,
              d->diag_slot, d->lrow_ptr, d->lrow_slot, d->lrow_col, d->urow_ptr,
-             d->urow_slot, d->urow_col, d->status};
+             d->urow_slot, d->urow_col, d->lu_rec,  d->lu_fast,   d->u_rec,
+             d->u_fast,    d->status};
 }
 static Lev lev_of(const DevLevels &L) {
   return Lev{L.level_chain_ptr, L.chain_col_ptr, L.cols,
@@ -224,6 +231,161 @@ __device__ __forceinline__ void ltsolve_col(const Pat &P, int j,
   y[(size_t)j * B + inst] = s;
 }
 
+
+// ------------------------------------------------ register-forwarding chains
+// Chains whose columns have <= 4 entries run from the 16-int records built by
+// compile_fast_programs: the column's values live in registers, operands that
+// the previous column of the chain produced are forwarded from registers, and
+// the next column's A and b values are loaded while the current one computes.
+// Each entry still sees the generic path's operation sequence (bit-identical).
+
+__device__ __forceinline__ double sel4(double v0, double v1, double v2,
+                                       double v3, int i) {
+  return i == 0 ? v0 : (i == 1 ? v1 : (i == 2 ? v2 : v3));
+}
+
+// A prefetched refactor column: record header and its A and b values.
+struct FastCol {
+  const int *r;
+  int slot0, col, meta;
+  double a0, a1, a2, a3, bb;
+};
+
+template <int W>
+__device__ __forceinline__ void fast_lu_load(FastCol &c, const int *r,
+                                             const double *A, const double *b,
+                                             size_t B, int inst) {
+  c.r = r;
+  c.slot0 = uniform<W>(r[0]);
+  c.col = uniform<W>(r[1]);
+  c.meta = uniform<W>(r[2]);
+  const int ne = c.meta & 7;
+  c.a0 = A[(size_t)c.slot0 * B + inst];
+  c.a1 = ne > 1 ? A[(size_t)(c.slot0 + 1) * B + inst] : 0.0;
+  c.a2 = ne > 2 ? A[(size_t)(c.slot0 + 2) * B + inst] : 0.0;
+  c.a3 = ne > 3 ? A[(size_t)(c.slot0 + 3) * B + inst] : 0.0;
+  c.bb = b[(size_t)c.col * B + inst];
+}
+
+template <int W>
+__device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
+                                              const double *A, double *LU,
+                                              const double *b, double *x,
+                                              size_t B, int inst) {
+  double p0 = 0, p1 = 0, p2 = 0, p3 = 0, yprev = 0;
+  // two columns in flight ahead of the one being computed
+  FastCol c0, c1, c2;
+  const int *rec = P.lu_rec;
+  fast_lu_load<W>(c0, rec + (size_t)t0 * 16, A, b, B, inst);
+  if (t0 + 1 < t1)
+    fast_lu_load<W>(c1, rec + (size_t)(t0 + 1) * 16, A, b, B, inst);
+  for (int t = t0; t < t1; ++t) {
+    if (t + 2 < t1)
+      fast_lu_load<W>(c2, rec + (size_t)(t + 2) * 16, A, b, B, inst);
+    const int *r = c0.r;
+    const int slot0 = c0.slot0, col = c0.col, meta = c0.meta, ne = meta & 7;
+    double v0 = c0.a0, v1 = c0.a1, v2 = c0.a2, v3 = c0.a3;
+    const int nops = (meta >> 6) & 7;
+    for (int k = 0; k < nops; ++k) {
+      const int lsrc = uniform<W>(r[4 + 2 * k]);
+      const int tu = uniform<W>(r[5 + 2 * k]);
+      const double l =
+          lsrc >= 0 ? LU[(size_t)lsrc * B + inst] : sel4(p0, p1, p2, p3, -1 - lsrc);
+      const double prod = l * sel4(v0, v1, v2, v3, tu >> 2);
+      switch (tu & 3) {
+      case 0: v0 = v0 - prod; break;
+      case 1: v1 = v1 - prod; break;
+      case 2: v2 = v2 - prod; break;
+      default: v3 = v3 - prod; break;
+      }
+    }
+    const int d = (meta >> 3) & 7;
+    const double piv = sel4(v0, v1, v2, v3, d);
+    if (fabs(piv) < 1e-8)
+      record_status(P.status, PVGLU_E_SINGULAR, col, inst);
+    if (d < 1 && ne > 1)
+      v1 = v1 / piv;
+    if (d < 2 && ne > 2)
+      v2 = v2 / piv;
+    if (d < 3 && ne > 3)
+      v3 = v3 / piv;
+    LU[(size_t)slot0 * B + inst] = v0;
+    if (ne > 1)
+      LU[(size_t)(slot0 + 1) * B + inst] = v1;
+    if (ne > 2)
+      LU[(size_t)(slot0 + 2) * B + inst] = v2;
+    if (ne > 3)
+      LU[(size_t)(slot0 + 3) * B + inst] = v3;
+    // forward substitution of row col
+    const int nlr = (meta >> 9) & 3;
+    double y = c0.bb;
+    for (int k = 0; k < nlr; ++k) {
+      const int lsrc = uniform<W>(r[12 + 2 * k]);
+      const int ysrc = uniform<W>(r[13 + 2 * k]);
+      const double l =
+          lsrc >= 0 ? LU[(size_t)lsrc * B + inst] : sel4(p0, p1, p2, p3, -1 - lsrc);
+      const double yj = ysrc >= 0 ? x[(size_t)ysrc * B + inst] : yprev;
+      y = y - l * yj;
+    }
+    x[(size_t)col * B + inst] = y;
+    yprev = y;
+    p0 = v0;
+    p1 = v1;
+    p2 = v2;
+    p3 = v3;
+    c0 = c1;
+    c1 = c2;
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void fast_u_chain(const Pat &P, int t0, int t1,
+                                             const double *LU, double *x,
+                                             size_t B, int inst) {
+  const int *r = P.u_rec + (size_t)t0 * 8;
+  int row = uniform<W>(r[0]), dslot = uniform<W>(r[1]), nt = uniform<W>(r[2]);
+  double y = x[(size_t)row * B + inst];
+  double dv = LU[(size_t)dslot * B + inst];
+  double u0 = nt > 0 ? LU[(size_t)uniform<W>(r[4]) * B + inst] : 0.0;
+  double u1 = nt > 1 ? LU[(size_t)uniform<W>(r[6]) * B + inst] : 0.0;
+  double xprev = 0;
+  for (int t = t0; t < t1; ++t) {
+    const int *nr = r + 8;
+    int nrow = 0, ndslot = 0, nnt = 0;
+    double ny = 0, ndv = 0, nu0 = 0, nu1 = 0;
+    if (t + 1 < t1) {
+      nrow = uniform<W>(nr[0]);
+      ndslot = uniform<W>(nr[1]);
+      nnt = uniform<W>(nr[2]);
+      ny = x[(size_t)nrow * B + inst];
+      ndv = LU[(size_t)ndslot * B + inst];
+      if (nnt > 0)
+        nu0 = LU[(size_t)uniform<W>(nr[4]) * B + inst];
+      if (nnt > 1)
+        nu1 = LU[(size_t)uniform<W>(nr[6]) * B + inst];
+    }
+    double s = y;
+    if (nt > 0) {
+      const int xs = uniform<W>(r[5]);
+      s = s - u0 * (xs >= 0 ? x[(size_t)xs * B + inst] : xprev);
+    }
+    if (nt > 1) {
+      const int xs = uniform<W>(r[7]);
+      s = s - u1 * (xs >= 0 ? x[(size_t)xs * B + inst] : xprev);
+    }
+    const double xi = s / dv;
+    x[(size_t)row * B + inst] = xi;
+    xprev = xi;
+    row = nrow;
+    nt = nnt;
+    y = ny;
+    dv = ndv;
+    u0 = nu0;
+    u1 = nu1;
+    r = nr;
+  }
+}
+
 enum Op { OP_REFACTOR = 0, OP_REFACTOR_FUSED, OP_LSOLVE, OP_USOLVE, OP_UT, OP_LT };
 
 template <int W, int OP>
@@ -233,6 +395,18 @@ __device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
                                           int inst) {
   const int t0 = uniform<W>(L.chain_col_ptr[chain]);
   const int t1 = uniform<W>(L.chain_col_ptr[chain + 1]);
+  if constexpr (OP == OP_REFACTOR_FUSED) {
+    if (uniform<W>(P.lu_fast[chain])) {
+      fast_lu_chain<W>(P, t0, t1, A, LU, b, x, B, inst);
+      return;
+    }
+  }
+  if constexpr (OP == OP_USOLVE) {
+    if (uniform<W>(P.u_fast[chain])) {
+      fast_u_chain<W>(P, t0, t1, LU, x, B, inst);
+      return;
+    }
+  }
   for (int t = t0; t < t1; ++t) {
     const int c = uniform<W>(L.cols[t]);
     if constexpr (OP == OP_REFACTOR)
@@ -261,8 +435,10 @@ __global__ void __launch_bounds__(256)
   const int lane = gid % W;
   if (group >= nchains * tiles)
     return;
-  const int tile = group / nchains;
-  const int chain = uniform<W>(L.level_chain_ptr[level]) + group % nchains;
+  // consecutive wavefronts take consecutive instance tiles of one chain, so
+  // the tiles of a slot row are streamed together
+  const int tile = group % tiles;
+  const int chain = uniform<W>(L.level_chain_ptr[level]) + group / tiles;
   const int inst = tile * W + lane;
   if (inst >= B)
     return;
@@ -305,12 +481,13 @@ __device__ __forceinline__ void component_levels(const Pat &P, const Lev &L,
 
 // mode 0: refactor only; 1: refactor + solve; 2: solve only; 3: transposed solve
 template <int W>
-__global__ void __launch_bounds__(1024)
-    k_component(Pat P, Lev Llu, Lev Ll, Lev Lu, Lev Llt, int mode, bool fuse,
-                int tiles, const double *A, double *LU, const double *b,
-                double *x, int B) {
-  const int comp = blockIdx.x / tiles;
-  const int tile = blockIdx.x % tiles;
+__device__ __forceinline__ void component_body(const Pat &P, const Lev &Llu,
+                                               const Lev &Ll, const Lev &Lu,
+                                               const Lev &Llt, int mode,
+                                               bool fuse, int comp, int tile,
+                                               const double *A, double *LU,
+                                               const double *b, double *x,
+                                               int B) {
   const int slots = blockDim.x / W;
   const int slot = uniform<W>((int)threadIdx.x / W);
   const int lane = threadIdx.x % W;
@@ -338,6 +515,43 @@ __global__ void __launch_bounds__(1024)
                              inst, OP_USOLVE);
 }
 
+template <int W>
+__global__ void __launch_bounds__(1024)
+    k_component(Pat P, Lev Llu, Lev Ll, Lev Lu, Lev Llt, int mode, bool fuse,
+                int tiles, const double *A, double *LU, const double *b,
+                double *x, int B) {
+  component_body<W>(P, Llu, Ll, Lu, Llt, mode, fuse, blockIdx.x / tiles,
+                    blockIdx.x % tiles, A, LU, b, x, B);
+}
+
+// Multi-pattern work list: groups of instances with their own plans in one
+// launch; block -> (group, component, instance tile).
+constexpr int kMaxGroups = 6;
+struct MultiArgs {
+  int ngroups;
+  int block_start[kMaxGroups + 1];
+  int tiles[kMaxGroups];
+  int B[kMaxGroups];
+  int fuse[kMaxGroups];
+  Pat pat[kMaxGroups];
+  Lev lu[kMaxGroups], l[kMaxGroups], u[kMaxGroups], lt[kMaxGroups];
+  const double *a[kMaxGroups];
+  double *luv[kMaxGroups];
+  const double *b[kMaxGroups];
+  double *x[kMaxGroups];
+};
+
+template <int W>
+__global__ void __launch_bounds__(1024) k_multi(MultiArgs m, int mode) {
+  int g = 0;
+  while (g + 1 < m.ngroups && (int)blockIdx.x >= m.block_start[g + 1])
+    ++g;
+  const int local = blockIdx.x - m.block_start[g];
+  component_body<W>(m.pat[g], m.lu[g], m.l[g], m.u[g], m.lt[g], mode,
+                    m.fuse[g] != 0, local / m.tiles[g], local % m.tiles[g],
+                    m.a[g], m.luv[g], m.b[g], m.x[g], m.B[g]);
+}
+
 __global__ void k_fill_int(int *p, int n, int v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n)
@@ -354,13 +568,16 @@ __global__ void __launch_bounds__(256)
     k_newton_update(int ne, int B, int per_thread, const int *__restrict__ pos,
                     const int *__restrict__ neg, const double *__restrict__ x,
                     double *__restrict__ v, double *delta_sum,
-                    unsigned long long *delta_max, int *status) {
+                    unsigned long long *delta_max, const int *active,
+                    int *status) {
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const long total = (long)((ne + per_thread - 1) / per_thread) * B;
   if (gid >= total)
     return;
   const int in = (int)(gid % B);
   const int chunk = (int)(gid / B);
+  if (active && !active[in])
+    return; // converged or stalled instance: frozen
   const int e0 = chunk * per_thread;
   const int e1 = min(ne, e0 + per_thread);
   double sum = 0.0, mx = 0.0;
@@ -493,6 +710,10 @@ extern "C" int pvglu_plan_upload(pvglu_plan *P) {
       (st = upload_levels(d, P->k_l, d->l)) ||
       (st = upload_levels(d, P->k_u, d->u)) ||
       (st = upload_levels(d, P->k_lt, d->lt)) ||
+      (st = dev_upload(d, &d->lu_rec, P->fast_lu_rec)) ||
+      (st = dev_upload(d, &d->lu_fast, P->fast_lu_chain)) ||
+      (st = dev_upload(d, &d->u_rec, P->fast_u_rec)) ||
+      (st = dev_upload(d, &d->u_fast, P->fast_u_chain)) ||
       (st = dev_upload(d, &d->status, std::vector<int>(4, 0)))) {
     device_plan_destroy(P);
     return st;
@@ -713,7 +934,7 @@ extern "C" int pvglu_newton_update(pvglu_plan *P, int ne, int batch,
                                    const int *pos, const int *neg,
                                    const double *x, double *v_elem,
                                    double *delta_sum, double *delta_max,
-                                   void *stream) {
+                                   const int *active, void *stream) {
   clear_error();
   int st = require_device(P);
   if (st)
@@ -731,7 +952,7 @@ extern "C" int pvglu_newton_update(pvglu_plan *P, int ne, int batch,
                      (hipStream_t)stream, (int *)delta_max, 2 * batch, 0);
   hipLaunchKernelGGL(k_newton_update, dim3(blocks), dim3(256), 0,
                      (hipStream_t)stream, ne, batch, per, pos, neg, x, v_elem,
-                     delta_sum, (unsigned long long *)delta_max,
+                     delta_sum, (unsigned long long *)delta_max, active,
                      P->dev->status);
   HIP_TRY(hipGetLastError());
   return PVGLU_OK;
@@ -863,4 +1084,116 @@ extern "C" int pvglu_autotune(pvglu_plan *P, int batch, int reps,
   if (best_ms)
     *best_ms = best;
   return PVGLU_OK;
+}
+
+// ------------------------------------------------------ multi-pattern batch
+
+struct pvglu_multi {
+  int ngroups = 0;
+  std::vector<pvglu_plan *> plans;
+  std::vector<int> batches;
+  int width = 16, threads = 256;
+};
+
+extern "C" int pvglu_multi_create(int ngroups, pvglu_plan *const *plans,
+                                  const int *batches, pvglu_multi **out) {
+  clear_error();
+  if (!out || ngroups <= 0 || ngroups > kMaxGroups || !plans || !batches)
+    return fail(PVGLU_E_ARG, "multi_create: 1..%d groups required", kMaxGroups);
+  auto *m = new pvglu_multi();
+  m->ngroups = ngroups;
+  int minB = 1 << 30, maxc = 1;
+  for (int g = 0; g < ngroups; g++) {
+    if (!plans[g] || batches[g] <= 0) {
+      delete m;
+      return fail(PVGLU_E_ARG, "multi_create: bad group %d", g);
+    }
+    int st = require_device(plans[g]);
+    if (st) {
+      delete m;
+      return st;
+    }
+    m->plans.push_back(plans[g]);
+    m->batches.push_back(batches[g]);
+    minB = std::min(minB, batches[g]);
+    const pvglu_plan *P = plans[g];
+    maxc = std::max({maxc, P->k_lu.max_comp_chains, P->k_u.max_comp_chains,
+                     P->k_l.max_comp_chains, P->k_lt.max_comp_chains});
+  }
+  m->width = minB >= 64 ? 16 : (minB >= 16 ? 8 : (minB >= 4 ? 4 : 1));
+  int slots = 1;
+  while (slots < maxc && slots * 2 * m->width <= 1024)
+    slots *= 2;
+  m->threads = std::min(1024, std::max(64, slots * m->width));
+  *out = m;
+  return PVGLU_OK;
+}
+
+extern "C" void pvglu_multi_destroy(pvglu_multi *m) { delete m; }
+
+template <int W>
+static int launch_multi_w(pvglu_multi *m, int mode, const double *const *a,
+                          double *const *lu, const double *const *b,
+                          double *const *x, hipStream_t s) {
+  MultiArgs args{};
+  args.ngroups = m->ngroups;
+  int blocks = 0;
+  for (int g = 0; g < m->ngroups; g++) {
+    pvglu_plan *P = m->plans[g];
+    const DevicePlan *d = P->dev;
+    const int tiles = (m->batches[g] + W - 1) / W;
+    args.block_start[g] = blocks;
+    blocks += P->ncomp * tiles;
+    args.tiles[g] = tiles;
+    args.B[g] = m->batches[g];
+    args.fuse[g] = P->fuse_lsolve ? 1 : 0;
+    args.pat[g] = pat_of(d);
+    args.lu[g] = lev_of(d->lu);
+    args.l[g] = lev_of(d->l);
+    args.u[g] = lev_of(d->u);
+    args.lt[g] = lev_of(d->lt);
+    args.a[g] = a ? a[g] : nullptr;
+    args.luv[g] = lu[g];
+    args.b[g] = b[g];
+    args.x[g] = x[g];
+  }
+  args.block_start[m->ngroups] = blocks;
+  hipLaunchKernelGGL((k_multi<W>), dim3(blocks), dim3(m->threads), 0, s, args,
+                     mode);
+  HIP_TRY(hipGetLastError());
+  return PVGLU_OK;
+}
+
+static int launch_multi(pvglu_multi *m, int mode, const double *const *a,
+                        double *const *lu, const double *const *b,
+                        double *const *x, void *stream) {
+  clear_error();
+  if (!m || !lu || !b || !x || (mode != 3 && !a))
+    return fail(PVGLU_E_ARG, "multi: null pointer arrays");
+  for (int g = 0; g < m->ngroups; g++)
+    if (!lu[g] || !b[g] || !x[g] || (mode != 3 && !a[g]))
+      return fail(PVGLU_E_ARG, "multi: null device pointer in group %d", g);
+  hipStream_t s = (hipStream_t)stream;
+  switch (m->width) {
+  case 1: return launch_multi_w<1>(m, mode, a, lu, b, x, s);
+  case 4: return launch_multi_w<4>(m, mode, a, lu, b, x, s);
+  case 8: return launch_multi_w<8>(m, mode, a, lu, b, x, s);
+  default: return launch_multi_w<16>(m, mode, a, lu, b, x, s);
+  }
+}
+
+extern "C" int pvglu_multi_refactor_solve(pvglu_multi *m,
+                                          const double *const *a,
+                                          double *const *lu,
+                                          const double *const *b,
+                                          double *const *x, void *stream) {
+  return launch_multi(m, 1, a, lu, b, x, stream);
+}
+
+extern "C" int pvglu_multi_solve_transpose(pvglu_multi *m,
+                                           const double *const *lu,
+                                           const double *const *g,
+                                           double *const *y, void *stream) {
+  return launch_multi(m, 3, nullptr, const_cast<double *const *>(lu), g, y,
+                      stream);
 }

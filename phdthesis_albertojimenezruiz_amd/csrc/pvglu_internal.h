@@ -74,6 +74,11 @@ struct pvglu_plan {
   int ncomp = 0;
   std::vector<int> comp_of; // component id per column
   pvglu::KernelLevels k_lu, k_l, k_u, k_lt;
+  // Register-forwarding programs (see compile_fast_programs): one 16-int
+  // record per refactor column and one 8-int record per backward-substitution
+  // row, in kernel order, plus a per-chain eligibility flag.
+  std::vector<int> fast_lu_rec, fast_lu_chain, fast_u_rec, fast_u_chain;
+  int fast_lu_chains = 0, fast_u_chains = 0;
   uint64_t hash = 0;
 
   // schedule choice
@@ -88,6 +93,9 @@ namespace pvglu {
 // Build the kernel views (component-contiguous chains) of a level set.
 void build_kernel_levels(const LevelSet &ls, const std::vector<int> &comp_of,
                          int ncomp, KernelLevels &out);
+// Small-column register programs for the refactor (with the fused forward
+// substitution) and the backward substitution.
+void compile_fast_programs(pvglu_plan *P);
 // Device teardown hook (defined in the HIP unit).
 void device_plan_destroy(pvglu_plan *plan);
 } // namespace pvglu

@@ -67,8 +67,13 @@ struct NewtonDev {
   // instance-side
   double *G = nullptr, *v = nullptr, *geq = nullptr, *ieq = nullptr,
          *A = nullptr, *LU = nullptr, *b = nullptr, *x = nullptr,
-         *dsum = nullptr, *dmax = nullptr, *gv = nullptr, *gP = nullptr,
-         *mu = nullptr, *dG = nullptr, *dgc = nullptr;
+         *dsum = nullptr, *dmax = nullptr, *gvs = nullptr, *gP = nullptr,
+         *mu = nullptr, *dG = nullptr, *dgc = nullptr, *lastmax = nullptr,
+         *ladder = nullptr;
+  int *stage = nullptr, *active = nullptr, *stall = nullptr, *iters = nullptr,
+      *remaining = nullptr;
+  int nstages = 12; // stages of the current solve (12 = ladder, 1 = last only)
+  double tol = 1e-10;    // captured into the graph; re-captured when changed
   hipStream_t s = nullptr;
   hipGraphExec_t exec = nullptr;
   hipGraph_t graph = nullptr;
@@ -125,7 +130,7 @@ __global__ void k_stamp_a(int nnz, int B, const double *__restrict__ base,
   if (gid >= (long)nnz * B)
     return;
   const int q = (int)(gid / B), inst = (int)(gid % B);
-  const double gv = *gvp;
+  const double gv = gvp[inst]; // homotopy conductance of this instance's stage
   double a = base[q];
   for (int k = ptr[q]; k < ptr[q + 1]; ++k) {
     const double s = (double)sgn[k];
@@ -202,6 +207,77 @@ __global__ void k_grad_cables(int nc, int B, const int *__restrict__ ra,
   const double dv = x[(size_t)ca[c] * B + inst] - x[(size_t)cb[c] * B + inst];
   const double dl = mu[(size_t)ra[c] * B + inst] - mu[(size_t)rb[c] * B + inst];
   dg[gid] = -dl * dv;
+}
+
+
+__global__ void k_fill_int_n(int *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    p[i] = 0;
+}
+
+__global__ void k_fill_double(double *p, int n, double v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    p[i] = v;
+}
+
+// Per-instance Newton state at the start of a solve.
+__global__ void k_newton_init(int B, double gv0, int *stage, double *gvs,
+                              int *active, int *stall, int *iters,
+                              double *lastmax) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= B)
+    return;
+  stage[k] = 0;
+  gvs[k] = gv0;
+  active[k] = 1;
+  stall[k] = 0;
+  iters[k] = 0;
+  lastmax[k] = 0.0;
+}
+
+// Per-instance control after each iteration: the reference's loop
+// (PV/000_PVPP_simulator.cpp:163-181) run independently for every instance.
+// Intermediate homotopy stages end when sum |delta| < 1e-6 n; the last stage
+// ends when max |delta| <= tol, or -- at the FP64 noise floor of the system --
+// after stall_max iterations with max |delta| <= noise.  Finished instances
+// are frozen (the update kernel skips them); `remaining` counts the others.
+__global__ void k_newton_control(int B, int nstages,
+                                 const double *__restrict__ ladder,
+                                 double stage_crit, double tol, double noise,
+                                 int stall_max, const double *__restrict__ dsum,
+                                 const double *__restrict__ dmax, int *stage,
+                                 double *gvs, int *active, int *stall,
+                                 int *iters, double *lastmax, int *remaining) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= B || !active[k])
+    return;
+  iters[k]++;
+  const int st = stage[k];
+  const double m = dmax[k];
+  lastmax[k] = m;
+  if (st < nstages - 1) {
+    if (dsum[k] < stage_crit) {
+      stage[k] = st + 1;
+      gvs[k] = ladder[st + 1];
+    }
+    atomicAdd(remaining, 1);
+    return;
+  }
+  if (!(m > tol)) { // converged (NaN is caught by the status word)
+    active[k] = 0;
+    return;
+  }
+  if (m <= noise) {
+    if (++stall[k] >= stall_max) {
+      active[k] = 0;
+      return;
+    }
+  } else {
+    stall[k] = 0;
+  }
+  atomicAdd(remaining, 1);
 }
 
 // ------------------------------------------------------------ host helpers
@@ -335,7 +411,9 @@ int newton_setup(pvglu_pv *pv) {
       (st = dupload(A, &d->rpos, rpos)) || (st = dupload(A, &d->rneg, rneg)) ||
       (st = dupload(A, &d->cab_ra, cra)) || (st = dupload(A, &d->cab_rb, crb)) ||
       (st = dupload(A, &d->cab_ca, cca)) || (st = dupload(A, &d->cab_cb, ccb)) ||
-      (st = dupload(A, &d->bus_c, busc)) || (st = dalloc(A, &d->gv, 1)))
+      (st = dupload(A, &d->bus_c, busc)) ||
+      (st = dupload(A, &d->ladder,
+                    std::vector<double>(kLadder, kLadder + 12))))
     return st;
   HIP_TRY(hipStreamCreate(&d->s));
   return PVGLU_OK;
@@ -366,7 +444,11 @@ int newton_alloc_instances(pvglu_pv *pv, int B) {
       (st = dalloc(I, &d->b, n * B)) || (st = dalloc(I, &d->x, n * B)) ||
       (st = dalloc(I, &d->dsum, (size_t)B)) || (st = dalloc(I, &d->dmax, (size_t)B)) ||
       (st = dalloc(I, &d->gP, n * B)) || (st = dalloc(I, &d->mu, n * B)) ||
-      (st = dalloc(I, &d->dG, nm * B)) || (st = dalloc(I, &d->dgc, nc * B)))
+      (st = dalloc(I, &d->dG, nm * B)) || (st = dalloc(I, &d->dgc, nc * B)) ||
+      (st = dalloc(I, &d->gvs, (size_t)B)) || (st = dalloc(I, &d->lastmax, (size_t)B)) ||
+      (st = dalloc(I, &d->stage, (size_t)B)) || (st = dalloc(I, &d->active, (size_t)B)) ||
+      (st = dalloc(I, &d->stall, (size_t)B)) || (st = dalloc(I, &d->iters, (size_t)B)) ||
+      (st = dalloc(I, &d->remaining, 1)))
     return st;
   d->B = B;
   return PVGLU_OK;
@@ -382,22 +464,35 @@ int launch_stamp(pvglu_pv *pv, int B) {
                      d->G, d->rsh, d->v, d->geq, d->ieq);
   hipLaunchKernelGGL(k_stamp_a, grid_for((long)nnz * B), dim3(256), 0, d->s,
                      nnz, B, d->base_a, d->a_ptr, d->a_mod, d->a_sgn, d->geq,
-                     d->gv, d->A);
+                     d->gvs, d->A);
   hipLaunchKernelGGL(k_stamp_b, grid_for((long)n * B), dim3(256), 0, d->s, n,
                      B, d->base_b, d->b_ptr, d->b_mod, d->b_sgn, d->ieq, d->b);
   HIP_TRY(hipGetLastError());
   return PVGLU_OK;
 }
 
+// Stall rule of the last stage: max |delta| <= kNoise for kStallMax iterations.
+constexpr double kNoise = 1e-8;
+constexpr int kStallMax = 20;
+
 int launch_iteration(pvglu_pv *pv, int B) {
   NewtonDev *d = pv->nd;
+  hipLaunchKernelGGL(k_fill_int_n, dim3(1), dim3(64), 0, d->s, d->remaining, 1);
   int st = launch_stamp(pv, B);
   if (st)
     return st;
   if ((st = pvglu_refactor_solve(pv->plan, d->A, d->LU, d->b, d->x, B, d->s)))
     return st;
-  return pvglu_newton_update(pv->plan, pv->nmod, B, d->pos, d->neg, d->x, d->v,
-                             d->dsum, d->dmax, d->s);
+  if ((st = pvglu_newton_update(pv->plan, pv->nmod, B, d->pos, d->neg, d->x,
+                                d->v, d->dsum, d->dmax, d->active, d->s)))
+    return st;
+  const double *ladder = d->ladder + (12 - d->nstages);
+  hipLaunchKernelGGL(k_newton_control, grid_for(B), dim3(256), 0, d->s, B,
+                     d->nstages, ladder, 1e-6 * pv->n, d->tol, kNoise, kStallMax,
+                     d->dsum, d->dmax, d->stage, d->gvs, d->active, d->stall,
+                     d->iters, d->lastmax, d->remaining);
+  HIP_TRY(hipGetLastError());
+  return PVGLU_OK;
 }
 
 int build_graph(pvglu_pv *pv, int B) {
@@ -444,42 +539,35 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
                          hipMemcpyHostToDevice, d->s));
   if ((st = pvglu_status_reset(pv->plan, d->s)))
     return st;
+  const int nst = ladder ? 12 : 1;
+  if (d->exec && (d->nstages != nst || d->tol != tol)) {
+    (void)hipGraphExecDestroy(d->exec);
+    (void)hipGraphDestroy(d->graph);
+    d->exec = nullptr;
+    d->graph = nullptr;
+  }
+  d->nstages = nst;
+  d->tol = tol;
+  hipLaunchKernelGGL(k_newton_init, grid_for(B), dim3(256), 0, d->s, B,
+                     kLadder[12 - nst], d->stage, d->gvs, d->active, d->stall,
+                     d->iters, d->lastmax);
   if ((st = build_graph(pv, B)))
     return st;
-  std::vector<double> hsum(B), hmax(B);
-  const int nst = ladder ? (int)(sizeof kLadder / sizeof kLadder[0]) : 1;
-  const double *stages = ladder ? kLadder : kLadder + 11;
-  int iters = 0;
-  double last_max = 0;
+  int iters = 0, remaining = 1;
   auto t0 = std::chrono::steady_clock::now();
-  for (int si = 0; si < nst; si++) {
-    const double gv = stages[si];
-    const bool last = si == nst - 1;
-    HIP_TRY(hipMemcpyAsync(d->gv, &gv, sizeof(double), hipMemcpyHostToDevice,
-                           d->s));
-    for (;;) {
-      if (iters >= max_iters)
-        return fail(PVGLU_E_NOCONV, "batched Newton: %d iterations", iters);
-      HIP_TRY(hipGraphLaunch(d->exec, d->s));
-      iters++;
-      HIP_TRY(hipMemcpyAsync(hsum.data(), d->dsum, sizeof(double) * B,
-                             hipMemcpyDeviceToHost, d->s));
-      HIP_TRY(hipMemcpyAsync(hmax.data(), d->dmax, sizeof(double) * B,
-                             hipMemcpyDeviceToHost, d->s));
-      HIP_TRY(hipStreamSynchronize(d->s));
-      double msum = 0, mmax = 0;
-      for (int k = 0; k < B; k++) {
-        msum = std::max(msum, hsum[k]);
-        mmax = std::max(mmax, hmax[k]);
-      }
-      last_max = mmax;
-      if (std::isnan(msum) || std::isnan(mmax))
-        return fail(PVGLU_E_NOCONV, "batched Newton: NaN update");
-      if (!last && msum < 1e-6 * pv->n)
-        break; // reference stage criterion (000:170-172)
-      if (last && mmax <= tol)
-        break;
+  while (remaining > 0) {
+    if (iters >= max_iters) {
+      // hand back the current iterate for diagnosis
+      (void)hipMemcpy(v_module, d->v, sizeof(double) * nm * B,
+                      hipMemcpyDeviceToHost);
+      return fail(PVGLU_E_NOCONV, "batched Newton: %d iterations, %d instances left",
+                  iters, remaining);
     }
+    HIP_TRY(hipGraphLaunch(d->exec, d->s));
+    iters++;
+    HIP_TRY(hipMemcpyAsync(&remaining, d->remaining, sizeof(int),
+                           hipMemcpyDeviceToHost, d->s));
+    HIP_TRY(hipStreamSynchronize(d->s));
   }
   auto t1 = std::chrono::steady_clock::now();
   int code = 0, col = 0, inst = 0;
@@ -491,11 +579,16 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
   HIP_TRY(hipMemcpy(v_module, d->v, sizeof(double) * nm * B,
                     hipMemcpyDeviceToHost));
   if (stats) {
+    std::vector<int> it(B);
+    std::vector<double> lm(B);
+    HIP_TRY(hipMemcpy(it.data(), d->iters, sizeof(int) * B, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(lm.data(), d->lastmax, sizeof(double) * B,
+                      hipMemcpyDeviceToHost));
     const double ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    stats[0] = iters;
+    stats[0] = *std::max_element(it.begin(), it.end());
     stats[1] = ms;
     stats[2] = ms / std::max(iters, 1);
-    stats[3] = last_max;
+    stats[3] = *std::max_element(lm.begin(), lm.end());
   }
   return PVGLU_OK;
 }
@@ -635,7 +728,7 @@ extern "C" int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g,
   for (int si = 0; si < nst; si++) {
     const bool last = si == nst - 1;
     long double diff = 1e10;
-    int stage_it = 0;
+    int stage_it = 0, stall = 0;
     for (;;) {
       if (fixed_iters > 0) {
         if (stage_it >= fixed_iters)
@@ -665,7 +758,7 @@ extern "C" int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g,
       if ((st = pvglu_refactor(P, d->a, d->lu, 1, nullptr)) ||
           (st = pvglu_solve(P, d->lu, d->b, d->x, 1, nullptr)) ||
           (st = pvglu_newton_update(P, nm, 1, d->pos, d->neg, d->x, d->v, d->dsum,
-                                    d->dmax, nullptr)))
+                                    d->dmax, nullptr, nullptr)))
         return st;
       double dsum = 0, dmax = 0;
       HIP_TRY(hipMemcpy(pv->circuit.v_iterate.data(), d->v, sizeof(double) * nm,
@@ -678,8 +771,13 @@ extern "C" int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g,
       diff = dsum;
       if (std::isnan(dsum))
         return fail(PVGLU_E_NOCONV, "drop-in Newton: NaN update");
-      if (fixed_iters <= 0 && last && dmax <= tol)
-        break;
+      if (fixed_iters <= 0 && last) {
+        if (dmax <= tol)
+          break;
+        stall = dmax <= kNoise ? stall + 1 : 0; // FP64 noise floor
+        if (stall >= kStallMax)
+          break;
+      }
     }
   }
   int code = 0, col = 0, inst = 0;
@@ -739,13 +837,28 @@ extern "C" int pvglu_pv_stamp(pvglu_pv *pv, int batch, const double *g,
     geq = d->geq;
     ieq = d->ieq;
   }
-  HIP_TRY(hipMemcpyAsync(d->gv, &g_virtual, sizeof(double), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_fill_double, grid_for(B), dim3(256), 0, s, d->gvs, B,
+                     g_virtual);
   hipLaunchKernelGGL(k_eval, grid_for((long)nm * B), dim3(256), 0, s, nm, B, g,
                      d->rsh, v, geq, ieq);
   hipLaunchKernelGGL(k_stamp_a, grid_for((long)nnz * B), dim3(256), 0, s, nnz,
-                     B, d->base_a, d->a_ptr, d->a_mod, d->a_sgn, geq, d->gv, a);
+                     B, d->base_a, d->a_ptr, d->a_mod, d->a_sgn, geq, d->gvs, a);
   hipLaunchKernelGGL(k_stamp_b, grid_for((long)n * B), dim3(256), 0, s, n, B,
                      d->base_b, d->b_ptr, d->b_mod, d->b_sgn, ieq, b);
   HIP_TRY(hipGetLastError());
+  return PVGLU_OK;
+}
+
+extern "C" int pvglu_pv_newton_info(const pvglu_pv *pv, int batch, int *iters,
+                                    double *last_max_delta) {
+  clear_error();
+  if (!pv || !pv->nd || batch <= 0 || batch > pv->nd->B)
+    return fail(PVGLU_E_STATE, "newton_info: no batched solve of this size");
+  if (iters)
+    HIP_TRY(hipMemcpy(iters, pv->nd->iters, sizeof(int) * batch,
+                      hipMemcpyDeviceToHost));
+  if (last_max_delta)
+    HIP_TRY(hipMemcpy(last_max_delta, pv->nd->lastmax, sizeof(double) * batch,
+                      hipMemcpyDeviceToHost));
   return PVGLU_OK;
 }

@@ -224,6 +224,106 @@ void build_kernel_levels(const LevelSet &ls, const std::vector<int> &comp_of,
   }
 }
 
+// Register programs.  A chain qualifies when every column has at most 4
+// entries, at most 4 updates and at most 2 forward-substitution terms (the
+// series strings of a PV plant are such chains: tridiagonal eliminations).
+// Refactor record (16 ints) for the column at kernel position t:
+//   [0] first CSC slot  [1] column  [2] ne | diag<<3 | nops<<6 | nlr<<9
+//   [3] unused          [4+2k] L operand of update k: >=0 CSC slot, <0 entry
+//   -1-e of the previous column in the chain (a register)
+//   [5+2k] target entry | U entry<<2 (U operands are entries of this column)
+//   [12+2k] L operand of forward term k (same encoding)
+//   [13+2k] y operand: >=0 row, -1 the previous column's y (a register)
+// Backward record (8 ints) for the row at kernel position t of the U levels:
+//   [0] row  [1] diagonal slot  [2] nterms  [3] unused
+//   [4+2k] U slot of term k  [5+2k] x operand: >=0 row, -1 previous row's x
+static constexpr int kFastMaxE = 4, kFastMaxOps = 4, kFastMaxLr = 2;
+
+void compile_fast_programs(pvglu_plan *P) {
+  const int n = P->n;
+  const KernelLevels &K = P->k_lu;
+  const int nch = (int)K.chain_col_ptr.size() - 1;
+  P->fast_lu_rec.assign((size_t)n * 16, 0);
+  P->fast_lu_chain.assign(std::max(nch, 1), 0);
+  P->fast_lu_chains = 0;
+  for (int ch = 0; ch < nch; ch++) {
+    bool ok = P->fuse_lsolve;
+    for (int t = K.chain_col_ptr[ch]; ok && t < K.chain_col_ptr[ch + 1]; t++) {
+      const int c = K.cols[t];
+      const int prev = t > K.chain_col_ptr[ch] ? K.cols[t - 1] : -1;
+      int *r = &P->fast_lu_rec[(size_t)t * 16];
+      const int q0 = P->cp[c], ne = P->cp[c + 1] - q0;
+      const int d = P->diag_slot[c] - q0;
+      if (ne > kFastMaxE) {
+        ok = false;
+        break;
+      }
+      int nops = 0;
+      for (int q = q0; q < q0 + ne && ok; q++)
+        for (int o = P->op_ptr[q]; o < P->op_ptr[q + 1]; o++) {
+          if (nops == kFastMaxOps) {
+            ok = false;
+            break;
+          }
+          const int sl = P->op_l[o], su = P->op_u[o];
+          int lsrc = sl;
+          if (prev >= 0 && sl >= P->cp[prev] && sl < P->cp[prev + 1])
+            lsrc = -1 - (sl - P->cp[prev]);
+          r[4 + 2 * nops] = lsrc;
+          r[5 + 2 * nops] = (q - q0) | ((su - q0) << 2);
+          nops++;
+        }
+      const int nlr = P->lrow_ptr[c + 1] - P->lrow_ptr[c];
+      if (!ok || nlr > kFastMaxLr) {
+        ok = false;
+        break;
+      }
+      for (int k = 0; k < nlr; k++) {
+        const int sl = P->lrow_slot[P->lrow_ptr[c] + k];
+        const int j = P->lrow_col[P->lrow_ptr[c] + k];
+        int lsrc = sl;
+        if (prev >= 0 && sl >= P->cp[prev] && sl < P->cp[prev + 1])
+          lsrc = -1 - (sl - P->cp[prev]);
+        r[12 + 2 * k] = lsrc;
+        r[13 + 2 * k] = (j == prev) ? -1 : j;
+      }
+      r[0] = q0;
+      r[1] = c;
+      r[2] = ne | (d << 3) | (nops << 6) | (nlr << 9);
+    }
+    P->fast_lu_chain[ch] = ok ? 1 : 0;
+    P->fast_lu_chains += ok;
+  }
+  const KernelLevels &U = P->k_u;
+  const int nchu = (int)U.chain_col_ptr.size() - 1;
+  P->fast_u_rec.assign((size_t)n * 8, 0);
+  P->fast_u_chain.assign(std::max(nchu, 1), 0);
+  P->fast_u_chains = 0;
+  for (int ch = 0; ch < nchu; ch++) {
+    bool ok = true;
+    for (int t = U.chain_col_ptr[ch]; ok && t < U.chain_col_ptr[ch + 1]; t++) {
+      const int i = U.cols[t];
+      const int prev = t > U.chain_col_ptr[ch] ? U.cols[t - 1] : -1;
+      const int nt = P->urow_ptr[i + 1] - P->urow_ptr[i];
+      if (nt > kFastMaxLr) {
+        ok = false;
+        break;
+      }
+      int *r = &P->fast_u_rec[(size_t)t * 8];
+      r[0] = i;
+      r[1] = P->diag_slot[i];
+      r[2] = nt;
+      for (int k = 0; k < nt; k++) {
+        const int j = P->urow_col[P->urow_ptr[i] + k];
+        r[4 + 2 * k] = P->urow_slot[P->urow_ptr[i] + k];
+        r[5 + 2 * k] = (j == prev) ? -1 : j;
+      }
+    }
+    P->fast_u_chain[ch] = ok ? 1 : 0;
+    P->fast_u_chains += ok;
+  }
+}
+
 } // namespace pvglu
 
 using namespace pvglu;
@@ -439,6 +539,7 @@ extern "C" int pvglu_plan_create(int n, const int *rowptr, const int *colidx,
   build_kernel_levels(P->l, P->comp_of, P->ncomp, P->k_l);
   build_kernel_levels(P->u, P->comp_of, P->ncomp, P->k_u);
   build_kernel_levels(P->lt, P->comp_of, P->ncomp, P->k_lt);
+  compile_fast_programs(P);
 
   uint64_t h = 1469598103934665603ull;
   h = fnv1a(h, &n, sizeof n);
