@@ -101,6 +101,8 @@ enum {
   PVGLU_INFO_NOPS,
   PVGLU_INFO_FUSED_LSOLVE,
   PVGLU_INFO_NCOMP,
+  PVGLU_INFO_FAST_LU_CHAINS, /* refactor chains on the register-forwarding path */
+  PVGLU_INFO_FAST_U_CHAINS,  /* backward-substitution chains on that path */
   PVGLU_INFO_COUNT
 };
 int pvglu_plan_info(const pvglu_plan *plan, int what);
@@ -136,8 +138,16 @@ int pvglu_plan_export(const pvglu_plan *plan, int which, int *out);
  *   PVGLU_SCHED_COMPONENT  one launch for everything: one workgroup per
  *                          (independent component, instance tile) runs all
  *                          levels with workgroup barriers
+ *   PVGLU_SCHED_HYBRID     wide levels one kernel each; every run of two or
+ *                          more consecutive narrow levels (at most 4 chains
+ *                          per component) one COMPONENT-style launch
  * width: instances handled side by side per chain (1..64, power of two). */
-enum { PVGLU_SCHED_AUTO = 0, PVGLU_SCHED_LEVEL = 1, PVGLU_SCHED_COMPONENT = 2 };
+enum {
+  PVGLU_SCHED_AUTO = 0,
+  PVGLU_SCHED_LEVEL = 1,
+  PVGLU_SCHED_COMPONENT = 2,
+  PVGLU_SCHED_HYBRID = 3
+};
 int pvglu_set_schedule(pvglu_plan *plan, int variant, int width);
 int pvglu_get_schedule(const pvglu_plan *plan, int *variant, int *width);
 

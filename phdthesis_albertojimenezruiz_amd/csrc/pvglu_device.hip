@@ -43,9 +43,11 @@ namespace pvglu {
 // Device mirror of a level set in kernel order.
 struct DevLevels {
   int *level_chain_ptr = nullptr, *chain_col_ptr = nullptr, *cols = nullptr;
-  int *comp_ptr = nullptr, *comp_lv_begin = nullptr, *comp_lv_end = nullptr;
+  int *comp_ptr = nullptr, *comp_lv_begin = nullptr, *comp_lv_end = nullptr,
+      *comp_lv_level = nullptr;
   int nlev = 0, max_comp_chains = 0;
   std::vector<int> h_level_chain_ptr; // host copy for launch sizing
+  std::vector<int> h_level_comp_chains; // widest component per level
 };
 
 struct DevicePlan {
@@ -55,8 +57,8 @@ struct DevicePlan {
       *lrow_slot = nullptr, *lrow_col = nullptr, *urow_ptr = nullptr,
       *urow_slot = nullptr, *urow_col = nullptr;
   DevLevels lu, l, u, lt;
-  int *lu_rec = nullptr, *lu_fast = nullptr, *u_rec = nullptr,
-      *u_fast = nullptr;
+  int *lu_rec = nullptr, *lu_run = nullptr, *lu_fast = nullptr,
+      *u_rec = nullptr, *u_run = nullptr, *u_fast = nullptr;
   int *status = nullptr; // [code, column, instance, pad]
   std::vector<void *> allocs;
 };
@@ -78,8 +80,10 @@ struct Pat {
   const int *__restrict__ urow_slot;
   const int *__restrict__ urow_col;
   const int *__restrict__ lu_rec;
+  const int *__restrict__ lu_run;
   const int *__restrict__ lu_fast;
   const int *__restrict__ u_rec;
+  const int *__restrict__ u_run;
   const int *__restrict__ u_fast;
   int *status;
 };
@@ -91,17 +95,19 @@ struct Lev {
   const int *__restrict__ comp_ptr;
   const int *__restrict__ comp_lv_begin;
   const int *__restrict__ comp_lv_end;
+  const int *__restrict__ comp_lv_level;
 };
 
 static Pat pat_of(const DevicePlan *d) {
   return Pat{d->cp,       d->ri,        d->op_ptr,    d->op_l,     d->op_u,
              d->diag_slot, d->lrow_ptr, d->lrow_slot, d->lrow_col, d->urow_ptr,
-             d->urow_slot, d->urow_col, d->lu_rec,  d->lu_fast,   d->u_rec,
-             d->u_fast,    d->status};
+             d->urow_slot, d->urow_col, d->lu_rec,  d->lu_run,    d->lu_fast,
+             d->u_rec,     d->u_run,    d->u_fast, d->status};
 }
 static Lev lev_of(const DevLevels &L) {
   return Lev{L.level_chain_ptr, L.chain_col_ptr, L.cols,
-             L.comp_ptr,        L.comp_lv_begin, L.comp_lv_end};
+             L.comp_ptr,        L.comp_lv_begin, L.comp_lv_end,
+             L.comp_lv_level};
 }
 
 template <int W> __device__ __forceinline__ int uniform(int v) {
@@ -123,6 +129,35 @@ __device__ __forceinline__ void record_status(int *status, int code, int col,
 // followed by the forward substitution of row c.
 // Operation order per entry = GLU_CSC_CPU_sequential's (PV/GLU_CSC_CPU_levels_strings.cpp:32-62):
 // v - L*U for ascending j, then the division by the pivot for i > c.
+// Operand loads are issued in chunks of kChunk independent loads, then the
+// chunk's updates are applied in order: a wide column (the DC bus: 24 updates
+// on the diagonal, a 24-term forward row) costs a few memory latencies instead
+// of one per update.
+constexpr int kChunk = 8;
+
+template <int W>
+__device__ __forceinline__ double dot_sub_chunked(double v, const int *sl_arr,
+                                                  const int *sr_arr,
+                                                  const double *L,
+                                                  const double *R, int k0,
+                                                  int k1, size_t B, int inst) {
+  for (int kb = k0; kb < k1; kb += kChunk) {
+    const int cnt = min(kChunk, k1 - kb);
+    double l[kChunk], r[kChunk];
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j)
+      if (j < cnt) {
+        l[j] = L[(size_t)uniform<W>(sl_arr[kb + j]) * B + inst];
+        r[j] = R[(size_t)uniform<W>(sr_arr[kb + j]) * B + inst];
+      }
+#pragma unroll
+    for (int j = 0; j < kChunk; ++j)
+      if (j < cnt)
+        v = v - l[j] * r[j];
+  }
+  return v;
+}
+
 template <int W, bool FUSE>
 __device__ __forceinline__ void refactor_column(const Pat &P, int c,
                                                 const double *A, double *LU,
@@ -132,33 +167,35 @@ __device__ __forceinline__ void refactor_column(const Pat &P, int c,
   const int q1 = uniform<W>(P.cp[c + 1]);
   const int qd = uniform<W>(P.diag[c]);
   double piv = 1.0;
-  for (int q = q0; q < q1; ++q) {
-    double v = A[(size_t)q * B + inst];
-    const int o0 = uniform<W>(P.op_ptr[q]);
-    const int o1 = uniform<W>(P.op_ptr[q + 1]);
-    for (int o = o0; o < o1; ++o) {
-      const int sl = uniform<W>(P.op_l[o]);
-      const int su = uniform<W>(P.op_u[o]);
-      v = v - LU[(size_t)sl * B + inst] * LU[(size_t)su * B + inst];
+  for (int qb = q0; qb < q1; qb += kChunk) {
+    const int cnt = min(kChunk, q1 - qb);
+    double a[kChunk];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k)
+      if (k < cnt)
+        a[k] = A[(size_t)(qb + k) * B + inst];
+#pragma unroll
+    for (int k = 0; k < kChunk; ++k) {
+      if (k >= cnt)
+        break;
+      const int q = qb + k;
+      double v = dot_sub_chunked<W>(a[k], P.op_l, P.op_u, LU, LU,
+                                    uniform<W>(P.op_ptr[q]),
+                                    uniform<W>(P.op_ptr[q + 1]), B, inst);
+      if (q == qd) {
+        piv = v;
+        if (fabs(v) < 1e-8)
+          record_status(P.status, PVGLU_E_SINGULAR, c, inst);
+      } else if (q > qd) {
+        v = v / piv;
+      }
+      LU[(size_t)q * B + inst] = v;
     }
-    if (q == qd) {
-      piv = v;
-      if (fabs(v) < 1e-8)
-        record_status(P.status, PVGLU_E_SINGULAR, c, inst);
-    } else if (q > qd) {
-      v = v / piv;
-    }
-    LU[(size_t)q * B + inst] = v;
   }
   if constexpr (FUSE) {
-    double s = b[(size_t)c * B + inst];
-    const int k0 = uniform<W>(P.lrow_ptr[c]);
-    const int k1 = uniform<W>(P.lrow_ptr[c + 1]);
-    for (int k = k0; k < k1; ++k) {
-      const int sl = uniform<W>(P.lrow_slot[k]);
-      const int j = uniform<W>(P.lrow_col[k]);
-      s = s - LU[(size_t)sl * B + inst] * x[(size_t)j * B + inst];
-    }
+    const double s = dot_sub_chunked<W>(
+        b[(size_t)c * B + inst], P.lrow_slot, P.lrow_col, LU, x,
+        uniform<W>(P.lrow_ptr[c]), uniform<W>(P.lrow_ptr[c + 1]), B, inst);
     x[(size_t)c * B + inst] = s;
   }
 }
@@ -169,15 +206,9 @@ template <int W>
 __device__ __forceinline__ void lsolve_row(const Pat &P, int i,
                                            const double *LU, const double *b,
                                            double *x, size_t B, int inst) {
-  double s = b[(size_t)i * B + inst];
-  const int k0 = uniform<W>(P.lrow_ptr[i]);
-  const int k1 = uniform<W>(P.lrow_ptr[i + 1]);
-  for (int k = k0; k < k1; ++k) {
-    const int sl = uniform<W>(P.lrow_slot[k]);
-    const int j = uniform<W>(P.lrow_col[k]);
-    s = s - LU[(size_t)sl * B + inst] * x[(size_t)j * B + inst];
-  }
-  x[(size_t)i * B + inst] = s;
+  x[(size_t)i * B + inst] = dot_sub_chunked<W>(
+      b[(size_t)i * B + inst], P.lrow_slot, P.lrow_col, LU, x,
+      uniform<W>(P.lrow_ptr[i]), uniform<W>(P.lrow_ptr[i + 1]), B, inst);
 }
 
 // Row i of the backward substitution: x_i = (y_i - sum_j U(i,j) x_j)/U(i,i)
@@ -187,16 +218,12 @@ template <int W>
 __device__ __forceinline__ void usolve_row(const Pat &P, int i,
                                            const double *LU, double *x,
                                            size_t B, int inst) {
-  double s = x[(size_t)i * B + inst];
-  const int k0 = uniform<W>(P.urow_ptr[i]);
-  const int k1 = uniform<W>(P.urow_ptr[i + 1]);
-  for (int k = k0; k < k1; ++k) {
-    const int su = uniform<W>(P.urow_slot[k]);
-    const int j = uniform<W>(P.urow_col[k]);
-    s = s - LU[(size_t)su * B + inst] * x[(size_t)j * B + inst];
-  }
   const int d = uniform<W>(P.diag[i]);
-  x[(size_t)i * B + inst] = s / LU[(size_t)d * B + inst];
+  const double dv = LU[(size_t)d * B + inst];
+  const double s = dot_sub_chunked<W>(
+      x[(size_t)i * B + inst], P.urow_slot, P.urow_col, LU, x,
+      uniform<W>(P.urow_ptr[i]), uniform<W>(P.urow_ptr[i + 1]), B, inst);
+  x[(size_t)i * B + inst] = s / dv;
 }
 
 // Adjoint, U^T z = g: z_j = (g_j - sum_{i<j} U(i,j) z_i) / U(j,j), the CSC
@@ -400,12 +427,36 @@ __device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
       fast_lu_chain<W>(P, t0, t1, A, LU, b, x, B, inst);
       return;
     }
+    // mixed chain: register runs between generic columns
+    for (int t = t0; t < t1;) {
+      const int e = uniform<W>(P.lu_run[t]);
+      if (e > t) {
+        fast_lu_chain<W>(P, t, e, A, LU, b, x, B, inst);
+        t = e;
+      } else {
+        refactor_column<W, true>(P, uniform<W>(L.cols[t]), A, LU, b, x, B,
+                                 inst);
+        ++t;
+      }
+    }
+    return;
   }
   if constexpr (OP == OP_USOLVE) {
     if (uniform<W>(P.u_fast[chain])) {
       fast_u_chain<W>(P, t0, t1, LU, x, B, inst);
       return;
     }
+    for (int t = t0; t < t1;) {
+      const int e = uniform<W>(P.u_run[t]);
+      if (e > t) {
+        fast_u_chain<W>(P, t, e, LU, x, B, inst);
+        t = e;
+      } else {
+        usolve_row<W>(P, uniform<W>(L.cols[t]), LU, x, B, inst);
+        ++t;
+      }
+    }
+    return;
   }
   for (int t = t0; t < t1; ++t) {
     const int c = uniform<W>(L.cols[t]);
@@ -455,9 +506,13 @@ __device__ __forceinline__ void component_levels(const Pat &P, const Lev &L,
                                                  bool active, const double *A,
                                                  double *LU, const double *b,
                                                  double *x, size_t B, int inst,
-                                                 int op) {
+                                                 int op, int lv_lo = 0,
+                                                 int lv_hi = 1 << 30) {
   const int e0 = L.comp_ptr[comp], e1 = L.comp_ptr[comp + 1];
   for (int e = e0; e < e1; ++e) {
+    const int lv = L.comp_lv_level[e];
+    if (lv < lv_lo || lv >= lv_hi)
+      continue; // block-uniform: no work, no barrier
     const int c0 = L.comp_lv_begin[e], c1 = L.comp_lv_end[e];
     if (active)
       for (int ch = c0 + slot; ch < c1; ch += slots) {
@@ -522,6 +577,31 @@ __global__ void __launch_bounds__(1024)
                 double *x, int B) {
   component_body<W>(P, Llu, Ll, Lu, Llt, mode, fuse, blockIdx.x / tiles,
                     blockIdx.x % tiles, A, LU, b, x, B);
+}
+
+// HYBRID schedule: a run of narrow levels (few chains) in one launch.  Block
+// -> (component, tile); the four level sets are walked in order
+// (refactor, forward, backward, transposed-L) over the given level ranges,
+// each with its operation, barriers between levels.
+struct LvRange {
+  int lo[4], hi[4], op[4];
+};
+
+template <int W>
+__global__ void __launch_bounds__(1024)
+    k_hybrid(Pat P, Lev Llu, Lev Ll, Lev Lu, Lev Llt, LvRange R, int tiles,
+             const double *A, double *LU, const double *b, double *x, int B) {
+  const int comp = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+  const int slots = blockDim.x / W;
+  const int slot = uniform<W>((int)threadIdx.x / W);
+  const int inst = tile * W + (int)threadIdx.x % W;
+  const bool active = inst < B;
+  const Lev *sets[4] = {&Llu, &Ll, &Lu, &Llt};
+  for (int k = 0; k < 4; ++k)
+    if (R.lo[k] < R.hi[k])
+      component_levels<W, false>(P, *sets[k], comp, slot, slots, active, A, LU,
+                                 b, x, (size_t)B, inst, R.op[k], R.lo[k],
+                                 R.hi[k]);
 }
 
 // Multi-pattern work list: groups of instances with their own plans in one
@@ -636,11 +716,17 @@ static int upload_levels(DevicePlan *d, const KernelLevels &k, DevLevels &L) {
       (st = dev_upload(d, &L.cols, k.cols)) ||
       (st = dev_upload(d, &L.comp_ptr, k.comp_ptr)) ||
       (st = dev_upload(d, &L.comp_lv_begin, k.comp_lv_begin)) ||
-      (st = dev_upload(d, &L.comp_lv_end, k.comp_lv_end)))
+      (st = dev_upload(d, &L.comp_lv_end, k.comp_lv_end)) ||
+      (st = dev_upload(d, &L.comp_lv_level, k.comp_lv_level)))
     return st;
   L.nlev = (int)k.level_chain_ptr.size() - 1;
   L.max_comp_chains = k.max_comp_chains;
   L.h_level_chain_ptr = k.level_chain_ptr;
+  L.h_level_comp_chains.assign(std::max(L.nlev, 0), 0);
+  for (size_t e = 0; e < k.comp_lv_level.size(); ++e) {
+    int &m = L.h_level_comp_chains[k.comp_lv_level[e]];
+    m = std::max(m, k.comp_lv_end[e] - k.comp_lv_begin[e]);
+  }
   return PVGLU_OK;
 }
 
@@ -712,6 +798,8 @@ extern "C" int pvglu_plan_upload(pvglu_plan *P) {
       (st = upload_levels(d, P->k_lt, d->lt)) ||
       (st = dev_upload(d, &d->lu_rec, P->fast_lu_rec)) ||
       (st = dev_upload(d, &d->lu_fast, P->fast_lu_chain)) ||
+      (st = dev_upload(d, &d->lu_run, P->fast_lu_run)) ||
+      (st = dev_upload(d, &d->u_run, P->fast_u_run)) ||
       (st = dev_upload(d, &d->u_rec, P->fast_u_rec)) ||
       (st = dev_upload(d, &d->u_fast, P->fast_u_chain)) ||
       (st = dev_upload(d, &d->status, std::vector<int>(4, 0)))) {
@@ -734,8 +822,8 @@ static void pick_default_schedule(pvglu_plan *P, int B, int &variant, int &W,
   if (!valid_width(W))
     W = B >= 64 ? 64 : (B >= 16 ? 16 : (B >= 4 ? 4 : 1));
   if (variant == PVGLU_SCHED_AUTO)
-    variant = (P->ncomp >= 8 && B > 1) ? PVGLU_SCHED_COMPONENT
-                                       : PVGLU_SCHED_LEVEL;
+    variant = (P->ncomp >= 8 && B < 256) ? PVGLU_SCHED_COMPONENT
+                                         : PVGLU_SCHED_HYBRID;
   // COMPONENT: chain slots per block = widest component level, capped so the
   // block has at most 1024 threads.
   int maxc = std::max({P->k_lu.max_comp_chains, P->k_u.max_comp_chains,
@@ -766,6 +854,118 @@ static int launch_levels_w(pvglu_plan *P, const DevLevels &L, const double *A,
   }
   HIP_TRY(hipGetLastError());
   return PVGLU_OK;
+}
+
+// One level with the operation chosen at run time (HYBRID schedule).
+template <int W>
+static void launch_one_level(pvglu_plan *P, const DevLevels &L, int lv, int op,
+                             const double *A, double *LU, const double *b,
+                             double *x, int B, hipStream_t s) {
+  const int tiles = (B + W - 1) / W;
+  const int nch = L.h_level_chain_ptr[lv + 1] - L.h_level_chain_ptr[lv];
+  const long threads = (long)nch * tiles * W;
+  const int blocks = (int)((threads + 255) / 256);
+  if (blocks == 0)
+    return;
+  const Pat pat = pat_of(P->dev);
+  const Lev lev = lev_of(L);
+#define PVGLU_LVL(OPC)                                                         \
+  hipLaunchKernelGGL((k_level<W, OPC>), dim3(blocks), dim3(256), 0, s, pat,    \
+                     lev, lv, nch, tiles, A, LU, b, x, B)
+  switch (op) {
+  case OP_REFACTOR: PVGLU_LVL(OP_REFACTOR); break;
+  case OP_REFACTOR_FUSED: PVGLU_LVL(OP_REFACTOR_FUSED); break;
+  case OP_LSOLVE: PVGLU_LVL(OP_LSOLVE); break;
+  case OP_USOLVE: PVGLU_LVL(OP_USOLVE); break;
+  case OP_UT: PVGLU_LVL(OP_UT); break;
+  default: PVGLU_LVL(OP_LT); break;
+  }
+#undef PVGLU_LVL
+}
+
+// Chains per component at or below which a level counts as narrow.
+constexpr int kNarrowChains = 4;
+
+template <int W>
+static int launch_hybrid_w(pvglu_plan *P, int mode, const double *A,
+                           double *LU, const double *b, double *x, int B,
+                           hipStream_t s) {
+  DevicePlan *d = P->dev;
+  const DevLevels *sets[4] = {&d->lu, &d->l, &d->u, &d->lt};
+  int ops[4] = {-1, -1, -1, -1};
+  const bool fused = mode == 1 && P->fuse_lsolve;
+  if (mode == 3) {
+    ops[0] = OP_UT;
+    ops[3] = OP_LT;
+  } else {
+    if (mode <= 1)
+      ops[0] = fused ? OP_REFACTOR_FUSED : OP_REFACTOR;
+    if (mode == 2 || (mode == 1 && !fused))
+      ops[1] = OP_LSOLVE;
+    if (mode >= 1)
+      ops[2] = OP_USOLVE;
+  }
+  struct Step {
+    int set, lv, width;
+  };
+  std::vector<Step> steps;
+  for (int k = 0; k < 4; ++k)
+    if (ops[k] >= 0)
+      for (int lv = 0; lv < sets[k]->nlev; ++lv)
+        steps.push_back({k, lv, sets[k]->h_level_comp_chains[lv]});
+  const int tiles = (B + W - 1) / W;
+  const Pat pat = pat_of(d);
+  for (size_t i = 0; i < steps.size();) {
+    size_t j = i;
+    int widest = 1;
+    while (j < steps.size() && steps[j].width <= kNarrowChains) {
+      widest = std::max(widest, steps[j].width);
+      ++j;
+    }
+    if (j - i < 2) {
+      const Step &st = steps[i];
+      launch_one_level<W>(P, *sets[st.set], st.lv, ops[st.set], A, LU, b, x, B,
+                          s);
+      ++i;
+      continue;
+    }
+    LvRange R;
+    for (int k = 0; k < 4; ++k) {
+      R.lo[k] = 0;
+      R.hi[k] = 0;
+      R.op[k] = ops[k] < 0 ? 0 : ops[k];
+    }
+    for (size_t t = i; t < j; ++t) {
+      const Step &st = steps[t];
+      if (R.lo[st.set] == R.hi[st.set])
+        R.lo[st.set] = st.lv;
+      R.hi[st.set] = st.lv + 1;
+    }
+    int slots = 1;
+    while (slots < widest)
+      slots *= 2;
+    const int threads = std::max(64, slots * W);
+    hipLaunchKernelGGL((k_hybrid<W>), dim3(P->ncomp * tiles), dim3(threads), 0,
+                       s, pat, lev_of(d->lu), lev_of(d->l), lev_of(d->u),
+                       lev_of(d->lt), R, tiles, A, LU, b, x, B);
+    i = j;
+  }
+  HIP_TRY(hipGetLastError());
+  return PVGLU_OK;
+}
+
+static int launch_hybrid(pvglu_plan *P, int W, int mode, const double *A,
+                         double *LU, const double *b, double *x, int B,
+                         hipStream_t s) {
+  switch (W) {
+  case 1: return launch_hybrid_w<1>(P, mode, A, LU, b, x, B, s);
+  case 2: return launch_hybrid_w<2>(P, mode, A, LU, b, x, B, s);
+  case 4: return launch_hybrid_w<4>(P, mode, A, LU, b, x, B, s);
+  case 8: return launch_hybrid_w<8>(P, mode, A, LU, b, x, B, s);
+  case 16: return launch_hybrid_w<16>(P, mode, A, LU, b, x, B, s);
+  case 32: return launch_hybrid_w<32>(P, mode, A, LU, b, x, B, s);
+  default: return launch_hybrid_w<64>(P, mode, A, LU, b, x, B, s);
+  }
 }
 
 template <int OP>
@@ -828,6 +1028,8 @@ static int run_numeric(pvglu_plan *P, int mode, const double *A, double *LU,
   pick_default_schedule(P, B, variant, W, threads);
   if (variant == PVGLU_SCHED_COMPONENT)
     return launch_component(P, W, mode, threads, A, LU, b, x, B, s);
+  if (variant == PVGLU_SCHED_HYBRID)
+    return launch_hybrid(P, W, mode, (double *)A, LU, b, x, B, s);
   DevicePlan *d = P->dev;
   if (mode == 3) {
     if ((st = launch_levels<OP_UT>(P, d->lu, W, nullptr, (double *)LU, b, x,
@@ -909,7 +1111,7 @@ extern "C" int pvglu_set_schedule(pvglu_plan *P, int variant, int width) {
   clear_error();
   if (!P)
     return fail(PVGLU_E_ARG, "set_schedule: null plan");
-  if (variant < PVGLU_SCHED_AUTO || variant > PVGLU_SCHED_COMPONENT)
+  if (variant < PVGLU_SCHED_AUTO || variant > PVGLU_SCHED_HYBRID)
     return fail(PVGLU_E_ARG, "set_schedule: unknown variant %d", variant);
   if (width != 0 && !valid_width(width))
     return fail(PVGLU_E_ARG, "set_schedule: width %d not a power of two <= 64",
@@ -1030,7 +1232,7 @@ extern "C" int pvglu_autotune(pvglu_plan *P, int batch, int reps,
   double best = 1e300;
   int bv = PVGLU_SCHED_LEVEL, bw = 1;
   const int widths[] = {1, 2, 4, 8, 16, 32, 64};
-  for (int variant = PVGLU_SCHED_LEVEL; variant <= PVGLU_SCHED_COMPONENT;
+  for (int variant = PVGLU_SCHED_LEVEL; variant <= PVGLU_SCHED_HYBRID;
        ++variant) {
     for (int w : widths) {
       if (w > 1 && w / 2 >= batch)

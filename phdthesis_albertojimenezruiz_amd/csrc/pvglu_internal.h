@@ -45,7 +45,7 @@ struct KernelLevels {
   // component view: comp_ptr[c]..comp_ptr[c+1] indexes comp_lv_* entries,
   // each a (chain_begin, chain_end) range of one level of component c.
   std::vector<int> comp_ptr;
-  std::vector<int> comp_lv_begin, comp_lv_end;
+  std::vector<int> comp_lv_begin, comp_lv_end, comp_lv_level;
   int max_comp_chains = 0; // widest level of any component
 };
 
@@ -76,8 +76,10 @@ struct pvglu_plan {
   pvglu::KernelLevels k_lu, k_l, k_u, k_lt;
   // Register-forwarding programs (see compile_fast_programs): one 16-int
   // record per refactor column and one 8-int record per backward-substitution
-  // row, in kernel order, plus a per-chain eligibility flag.
-  std::vector<int> fast_lu_rec, fast_lu_chain, fast_u_rec, fast_u_chain;
+  // row, in kernel order, the end of the register run holding each position
+  // (0: generic column) and a per-chain whole-chain flag.
+  std::vector<int> fast_lu_rec, fast_lu_run, fast_lu_chain;
+  std::vector<int> fast_u_rec, fast_u_run, fast_u_chain;
   int fast_lu_chains = 0, fast_u_chains = 0;
   uint64_t hash = 0;
 

@@ -106,10 +106,12 @@ void newton_dev_destroy(NewtonDev *d) {
 __global__ void k_eval(int nm, int B, const double *__restrict__ Girr,
                        const double *__restrict__ rsh,
                        const double *__restrict__ v, double *__restrict__ geq,
-                       double *__restrict__ ieq) {
+                       double *__restrict__ ieq, const int *active) {
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long)nm * B)
     return;
+  if (active && !active[gid % B])
+    return; // frozen instance: its system (and so its solution) stays as is
   const int e = (int)(gid / B);
   const double V = v[gid];
   const ModuleEval me = module_current(V, kIphPerIrradiance * Girr[gid], rsh[e]);
@@ -125,11 +127,14 @@ __global__ void k_stamp_a(int nnz, int B, const double *__restrict__ base,
                           const int *__restrict__ mod,
                           const int *__restrict__ sgn,
                           const double *__restrict__ geq,
-                          const double *__restrict__ gvp, double *A) {
+                          const double *__restrict__ gvp, double *A,
+                          const int *active) {
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long)nnz * B)
     return;
   const int q = (int)(gid / B), inst = (int)(gid % B);
+  if (active && !active[inst])
+    return;
   const double gv = gvp[inst]; // homotopy conductance of this instance's stage
   double a = base[q];
   for (int k = ptr[q]; k < ptr[q + 1]; ++k) {
@@ -146,11 +151,14 @@ __global__ void k_stamp_b(int n, int B, const double *__restrict__ base,
                           const int *__restrict__ ptr,
                           const int *__restrict__ mod,
                           const int *__restrict__ sgn,
-                          const double *__restrict__ ieq, double *b) {
+                          const double *__restrict__ ieq, double *b,
+                          const int *active) {
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid >= (long)n * B)
     return;
   const int r = (int)(gid / B), inst = (int)(gid % B);
+  if (active && !active[inst])
+    return;
   double v = base[r];
   for (int k = ptr[r]; k < ptr[r + 1]; ++k)
     v = v - (double)sgn[k] * ieq[(size_t)mod[k] * B + inst];
@@ -457,16 +465,17 @@ int newton_alloc_instances(pvglu_pv *pv, int B) {
 inline dim3 grid_for(long count) { return dim3((unsigned)((count + 255) / 256)); }
 
 // eval + stamps on stream s
-int launch_stamp(pvglu_pv *pv, int B) {
+int launch_stamp(pvglu_pv *pv, int B, const int *active) {
   NewtonDev *d = pv->nd;
   const int nm = pv->nmod, n = pv->n, nnz = pv->plan->nnz;
   hipLaunchKernelGGL(k_eval, grid_for((long)nm * B), dim3(256), 0, d->s, nm, B,
-                     d->G, d->rsh, d->v, d->geq, d->ieq);
+                     d->G, d->rsh, d->v, d->geq, d->ieq, active);
   hipLaunchKernelGGL(k_stamp_a, grid_for((long)nnz * B), dim3(256), 0, d->s,
                      nnz, B, d->base_a, d->a_ptr, d->a_mod, d->a_sgn, d->geq,
-                     d->gvs, d->A);
+                     d->gvs, d->A, active);
   hipLaunchKernelGGL(k_stamp_b, grid_for((long)n * B), dim3(256), 0, d->s, n,
-                     B, d->base_b, d->b_ptr, d->b_mod, d->b_sgn, d->ieq, d->b);
+                     B, d->base_b, d->b_ptr, d->b_mod, d->b_sgn, d->ieq, d->b,
+                     active);
   HIP_TRY(hipGetLastError());
   return PVGLU_OK;
 }
@@ -478,7 +487,7 @@ constexpr int kStallMax = 20;
 int launch_iteration(pvglu_pv *pv, int B) {
   NewtonDev *d = pv->nd;
   hipLaunchKernelGGL(k_fill_int_n, dim3(1), dim3(64), 0, d->s, d->remaining, 1);
-  int st = launch_stamp(pv, B);
+  int st = launch_stamp(pv, B, d->active);
   if (st)
     return st;
   if ((st = pvglu_refactor_solve(pv->plan, d->A, d->LU, d->b, d->x, B, d->s)))
@@ -646,7 +655,7 @@ extern "C" int pvglu_pv_gradients(pvglu_pv *pv, int batch, const double *g,
   const int B = batch, n = pv->n, nm = pv->nmod, nc = pv->ncab;
   // Jacobian at the converged element voltages: stamp + refactor (the
   // companion matrix at V_e = V_branch(x) is dF/dx), then the last solve's x
-  if ((st = launch_stamp(pv, B)))
+  if ((st = launch_stamp(pv, B, nullptr)))
     return st;
   if ((st = pvglu_refactor_solve(pv->plan, d->A, d->LU, d->b, d->x, B, d->s)))
     return st;
@@ -840,11 +849,12 @@ extern "C" int pvglu_pv_stamp(pvglu_pv *pv, int batch, const double *g,
   hipLaunchKernelGGL(k_fill_double, grid_for(B), dim3(256), 0, s, d->gvs, B,
                      g_virtual);
   hipLaunchKernelGGL(k_eval, grid_for((long)nm * B), dim3(256), 0, s, nm, B, g,
-                     d->rsh, v, geq, ieq);
+                     d->rsh, v, geq, ieq, nullptr);
   hipLaunchKernelGGL(k_stamp_a, grid_for((long)nnz * B), dim3(256), 0, s, nnz,
-                     B, d->base_a, d->a_ptr, d->a_mod, d->a_sgn, geq, d->gvs, a);
+                     B, d->base_a, d->a_ptr, d->a_mod, d->a_sgn, geq, d->gvs, a,
+                     nullptr);
   hipLaunchKernelGGL(k_stamp_b, grid_for((long)n * B), dim3(256), 0, s, n, B,
-                     d->base_b, d->b_ptr, d->b_mod, d->b_sgn, ieq, b);
+                     d->base_b, d->b_ptr, d->b_mod, d->b_sgn, ieq, b, nullptr);
   HIP_TRY(hipGetLastError());
   return PVGLU_OK;
 }

@@ -215,113 +215,139 @@ void build_kernel_levels(const LevelSet &ls, const std::vector<int> &comp_of,
   out.comp_ptr.assign(ncomp + 1, 0);
   out.comp_lv_begin.clear();
   out.comp_lv_end.clear();
+  out.comp_lv_level.clear();
   for (int c = 0; c < ncomp; c++) {
     for (auto &e : per[c]) {
       out.comp_lv_begin.push_back(e[1]);
       out.comp_lv_end.push_back(e[2]);
+      out.comp_lv_level.push_back(e[0]);
     }
     out.comp_ptr[c + 1] = (int)out.comp_lv_begin.size();
   }
 }
 
-// Register programs.  A chain qualifies when every column has at most 4
-// entries, at most 4 updates and at most 2 forward-substitution terms (the
-// series strings of a PV plant are such chains: tridiagonal eliminations).
+// Register programs.  A column qualifies when it has at most 4 entries, at
+// most 4 updates and at most 2 forward-substitution terms (the series strings
+// of a PV plant: tridiagonal eliminations).  A maximal run of qualifying
+// consecutive columns of a chain runs on the register path; the other columns
+// of the chain (a DC bus, say) run the generic column code between runs.
 // Refactor record (16 ints) for the column at kernel position t:
 //   [0] first CSC slot  [1] column  [2] ne | diag<<3 | nops<<6 | nlr<<9
 //   [3] unused          [4+2k] L operand of update k: >=0 CSC slot, <0 entry
-//   -1-e of the previous column in the chain (a register)
+//   -1-e of the previous column of the run (a register)
 //   [5+2k] target entry | U entry<<2 (U operands are entries of this column)
 //   [12+2k] L operand of forward term k (same encoding)
 //   [13+2k] y operand: >=0 row, -1 the previous column's y (a register)
 // Backward record (8 ints) for the row at kernel position t of the U levels:
 //   [0] row  [1] diagonal slot  [2] nterms  [3] unused
 //   [4+2k] U slot of term k  [5+2k] x operand: >=0 row, -1 previous row's x
+// run[t] = end position of the register run that contains t (0 if t does not
+// qualify); chain flag = 1 when one run covers the whole chain.
 static constexpr int kFastMaxE = 4, kFastMaxOps = 4, kFastMaxLr = 2;
 
-void compile_fast_programs(pvglu_plan *P) {
-  const int n = P->n;
-  const KernelLevels &K = P->k_lu;
+static bool lu_column_qualifies(const pvglu_plan *P, int c) {
+  const int q0 = P->cp[c], ne = P->cp[c + 1] - q0;
+  if (ne > kFastMaxE || P->lrow_ptr[c + 1] - P->lrow_ptr[c] > kFastMaxLr)
+    return false;
+  return P->op_ptr[q0 + ne] - P->op_ptr[q0] <= kFastMaxOps;
+}
+
+static void encode_lu_record(const pvglu_plan *P, int c, int prev, int *r) {
+  const int q0 = P->cp[c], ne = P->cp[c + 1] - q0;
+  const int d = P->diag_slot[c] - q0;
+  auto lsrc_of = [&](int sl) {
+    if (prev >= 0 && sl >= P->cp[prev] && sl < P->cp[prev + 1])
+      return -1 - (sl - P->cp[prev]);
+    return sl;
+  };
+  int nops = 0;
+  for (int q = q0; q < q0 + ne; q++)
+    for (int o = P->op_ptr[q]; o < P->op_ptr[q + 1]; o++, nops++) {
+      r[4 + 2 * nops] = lsrc_of(P->op_l[o]);
+      r[5 + 2 * nops] = (q - q0) | ((P->op_u[o] - q0) << 2);
+    }
+  const int nlr = P->lrow_ptr[c + 1] - P->lrow_ptr[c];
+  for (int k = 0; k < nlr; k++) {
+    const int j = P->lrow_col[P->lrow_ptr[c] + k];
+    r[12 + 2 * k] = lsrc_of(P->lrow_slot[P->lrow_ptr[c] + k]);
+    r[13 + 2 * k] = (j == prev) ? -1 : j;
+  }
+  r[0] = q0;
+  r[1] = c;
+  r[2] = ne | (d << 3) | (nops << 6) | (nlr << 9);
+}
+
+static void encode_u_record(const pvglu_plan *P, int i, int prev, int *r) {
+  const int nt = P->urow_ptr[i + 1] - P->urow_ptr[i];
+  r[0] = i;
+  r[1] = P->diag_slot[i];
+  r[2] = nt;
+  for (int k = 0; k < nt; k++) {
+    const int j = P->urow_col[P->urow_ptr[i] + k];
+    r[4 + 2 * k] = P->urow_slot[P->urow_ptr[i] + k];
+    r[5 + 2 * k] = (j == prev) ? -1 : j;
+  }
+}
+
+// Shared run splitting: positions of each chain are grouped into maximal runs
+// of qualifying columns; records are encoded with the previous column of the
+// run (none for a run's first column).
+template <class Q, class E>
+static int compile_runs(const KernelLevels &K, int n, int rec_ints, Q qualifies,
+                        E encode, std::vector<int> &rec, std::vector<int> &run,
+                        std::vector<int> &chain_flag) {
   const int nch = (int)K.chain_col_ptr.size() - 1;
-  P->fast_lu_rec.assign((size_t)n * 16, 0);
-  P->fast_lu_chain.assign(std::max(nch, 1), 0);
-  P->fast_lu_chains = 0;
+  rec.assign((size_t)std::max(n, 1) * rec_ints, 0);
+  run.assign(std::max(n, 1), 0);
+  chain_flag.assign(std::max(nch, 1), 0);
+  int whole = 0;
   for (int ch = 0; ch < nch; ch++) {
-    bool ok = P->fuse_lsolve;
-    for (int t = K.chain_col_ptr[ch]; ok && t < K.chain_col_ptr[ch + 1]; t++) {
-      const int c = K.cols[t];
-      const int prev = t > K.chain_col_ptr[ch] ? K.cols[t - 1] : -1;
-      int *r = &P->fast_lu_rec[(size_t)t * 16];
-      const int q0 = P->cp[c], ne = P->cp[c + 1] - q0;
-      const int d = P->diag_slot[c] - q0;
-      if (ne > kFastMaxE) {
-        ok = false;
-        break;
+    const int t0 = K.chain_col_ptr[ch], t1 = K.chain_col_ptr[ch + 1];
+    int t = t0;
+    while (t < t1) {
+      if (!qualifies(K.cols[t])) {
+        ++t;
+        continue;
       }
-      int nops = 0;
-      for (int q = q0; q < q0 + ne && ok; q++)
-        for (int o = P->op_ptr[q]; o < P->op_ptr[q + 1]; o++) {
-          if (nops == kFastMaxOps) {
-            ok = false;
-            break;
-          }
-          const int sl = P->op_l[o], su = P->op_u[o];
-          int lsrc = sl;
-          if (prev >= 0 && sl >= P->cp[prev] && sl < P->cp[prev + 1])
-            lsrc = -1 - (sl - P->cp[prev]);
-          r[4 + 2 * nops] = lsrc;
-          r[5 + 2 * nops] = (q - q0) | ((su - q0) << 2);
-          nops++;
-        }
-      const int nlr = P->lrow_ptr[c + 1] - P->lrow_ptr[c];
-      if (!ok || nlr > kFastMaxLr) {
-        ok = false;
-        break;
+      int e = t;
+      while (e < t1 && qualifies(K.cols[e]))
+        ++e;
+      for (int u = t; u < e; u++) {
+        encode(K.cols[u], u > t ? K.cols[u - 1] : -1,
+               &rec[(size_t)u * rec_ints]);
+        run[u] = e;
       }
-      for (int k = 0; k < nlr; k++) {
-        const int sl = P->lrow_slot[P->lrow_ptr[c] + k];
-        const int j = P->lrow_col[P->lrow_ptr[c] + k];
-        int lsrc = sl;
-        if (prev >= 0 && sl >= P->cp[prev] && sl < P->cp[prev + 1])
-          lsrc = -1 - (sl - P->cp[prev]);
-        r[12 + 2 * k] = lsrc;
-        r[13 + 2 * k] = (j == prev) ? -1 : j;
+      if (t == t0 && e == t1) {
+        chain_flag[ch] = 1;
+        ++whole;
       }
-      r[0] = q0;
-      r[1] = c;
-      r[2] = ne | (d << 3) | (nops << 6) | (nlr << 9);
+      t = e;
     }
-    P->fast_lu_chain[ch] = ok ? 1 : 0;
-    P->fast_lu_chains += ok;
   }
-  const KernelLevels &U = P->k_u;
-  const int nchu = (int)U.chain_col_ptr.size() - 1;
-  P->fast_u_rec.assign((size_t)n * 8, 0);
-  P->fast_u_chain.assign(std::max(nchu, 1), 0);
-  P->fast_u_chains = 0;
-  for (int ch = 0; ch < nchu; ch++) {
-    bool ok = true;
-    for (int t = U.chain_col_ptr[ch]; ok && t < U.chain_col_ptr[ch + 1]; t++) {
-      const int i = U.cols[t];
-      const int prev = t > U.chain_col_ptr[ch] ? U.cols[t - 1] : -1;
-      const int nt = P->urow_ptr[i + 1] - P->urow_ptr[i];
-      if (nt > kFastMaxLr) {
-        ok = false;
-        break;
-      }
-      int *r = &P->fast_u_rec[(size_t)t * 8];
-      r[0] = i;
-      r[1] = P->diag_slot[i];
-      r[2] = nt;
-      for (int k = 0; k < nt; k++) {
-        const int j = P->urow_col[P->urow_ptr[i] + k];
-        r[4 + 2 * k] = P->urow_slot[P->urow_ptr[i] + k];
-        r[5 + 2 * k] = (j == prev) ? -1 : j;
-      }
-    }
-    P->fast_u_chain[ch] = ok ? 1 : 0;
-    P->fast_u_chains += ok;
+  return whole;
+}
+
+void compile_fast_programs(pvglu_plan *P) {
+  if (P->fuse_lsolve) {
+    P->fast_lu_chains = compile_runs(
+        P->k_lu, P->n, 16,
+        [&](int c) { return lu_column_qualifies(P, c); },
+        [&](int c, int prev, int *r) { encode_lu_record(P, c, prev, r); },
+        P->fast_lu_rec, P->fast_lu_run, P->fast_lu_chain);
+  } else {
+    // the register program computes the fused forward substitution
+    P->fast_lu_rec.assign((size_t)std::max(P->n, 1) * 16, 0);
+    P->fast_lu_run.assign(std::max(P->n, 1), 0);
+    P->fast_lu_chain.assign(std::max((int)P->k_lu.chain_col_ptr.size() - 1, 1), 0);
+    P->fast_lu_chains = 0;
   }
+  P->fast_u_chains = compile_runs(
+      P->k_u, P->n, 8,
+      [&](int i) {
+        return P->urow_ptr[i + 1] - P->urow_ptr[i] <= kFastMaxLr;
+      },
+      [&](int i, int prev, int *r) { encode_u_record(P, i, prev, r); },
+      P->fast_u_rec, P->fast_u_run, P->fast_u_chain);
 }
 
 } // namespace pvglu
@@ -574,6 +600,8 @@ extern "C" int pvglu_plan_info(const pvglu_plan *P, int what) {
   case PVGLU_INFO_NOPS: return (int)P->op_l.size();
   case PVGLU_INFO_FUSED_LSOLVE: return P->fuse_lsolve ? 1 : 0;
   case PVGLU_INFO_NCOMP: return P->ncomp;
+  case PVGLU_INFO_FAST_LU_CHAINS: return P->fast_lu_chains;
+  case PVGLU_INFO_FAST_U_CHAINS: return P->fast_u_chains;
   default: return fail(PVGLU_E_ARG, "plan_info: unknown field %d", what);
   }
 }

@@ -70,9 +70,12 @@ NOISE_AGREE = 2e-9
 
 
 def _agree(plant, x, v, xo, vo, gpu_last, oracle_last):
+    """Both sides stop once their last update is <= TOL, so each is within
+    about TOL of the fixed point: they agree to 2 TOL (NOISE_AGREE when an
+    instance ended at the noise floor)."""
     dx = np.abs(x - xo).max()
     both = gpu_last <= TOL and oracle_last <= TOL
-    assert dx <= (TOL if both else NOISE_AGREE), (dx, gpu_last, oracle_last)
+    assert dx <= (2 * TOL if both else NOISE_AGREE), (dx, gpu_last, oracle_last)
     return dx
 
 
@@ -83,11 +86,17 @@ def test_batched_cfg0_clouds_match_oracle(cfg0, ladder):
     x, v, p, stats = cfg0.newton_batched(G, tol=TOL, ladder=ladder)
     its, last = cfg0.newton_info(B)
     on = _oracle(cfg0)
+    buses = cfg0.buses()
     worst = 0.0
     for k in range(B):
         xo, vo, it = on.solve(9e-3 * G[:, k], cfg0.v0(), ladder=ladder, tol=TOL)
         worst = max(worst, _agree(cfg0, x[:, k], v[:, k], xo, vo, last[k], on.last_max))
-        assert p[k] == pytest.approx(on.power(xo, cfg0.buses()), rel=1e-11)
+        # device power reduction = the power formula on the device's x
+        assert p[k] == pytest.approx(on.power(x[:, k], buses), rel=1e-12)
+        # against the oracle's power: dP/dV_bus = G_N (2 V_bus - 600) ~ 6e6 W/V
+        # turns the voltage agreement into the power bound
+        bound = np.sum(1e4 * np.abs(2 * xo[buses] - 600.0)) * np.abs(x[:, k] - xo).max()
+        assert abs(p[k] - on.power(xo, buses)) <= bound + 1e-9 * abs(p[k])
     print(f"batched cfg0 (ladder={ladder}): iterations {its.tolist()}, "
           f"final max|dV| {last.max():.1e}, worst |dx| vs oracle {worst:.1e} V")
 
@@ -110,7 +119,7 @@ def test_batched_cfg1_matches_dropin_and_oracle(dev):
     x, v, p, stats = plant.newton_batched(G, tol=TOL, ladder=False)
     its, last = plant.newton_info(B)
     xd, vd, _ = plant.newton_dropin(G[:, 1], tol=TOL, ladder=False)
-    assert np.abs(x[:, 1] - xd).max() <= (TOL if last[1] <= TOL else NOISE_AGREE)
+    assert np.abs(x[:, 1] - xd).max() <= (2 * TOL if last[1] <= TOL else NOISE_AGREE)
     on = _oracle(plant)
     xo, vo, _ = on.solve(9e-3 * G[:, 2], plant.v0(), ladder=False, tol=TOL)
     dx = _agree(plant, x[:, 2], v[:, 2], xo, vo, last[2], on.last_max)
@@ -151,25 +160,58 @@ def test_gradients_against_oracle_adjoint_and_fd(cfg0):
     want_dgc = -(lam[a] - lam[b]) * (xo[a] - xo[b])
     assert np.abs(dgc[:, k] - want_dgc).max() <= 1e-9 * np.abs(want_dgc).max()
 
-    # Richardson-extrapolated central differences of the owning inverter's
-    # power (other inverters are decoupled): O(h^4) truncation, noise set by
-    # the 1e-13 V Newton tolerance times dP/dV_bus ~ 6e6 W/V.
+    # Central differences on the oracle Newton.  The owning inverter's power
+    # is taken from its strings' bottom-module currents (equal to
+    # V_bus G_N (V_bus - 600) at the solution by KCL, but free of the
+    # cancellation in V_bus - 600 that puts ~1e-6 W of noise on P).
+    # Modules: Richardson on h = 0.1, 0.05 W/m^2; bound 1e-6 relative plus an
+    # absolute 1e-8 W per W/m^2 (the FD noise of small derivatives).
+    # Cables: Richardson on h = 0.02, 0.01 S; 1e-6 relative plus 3e-8 W/S
+    # (the extrapolated differences of four cables of this plant agree with the
+    # oracle adjoint to <= 9e-9 W/S; h = 1e-4 S leaves 2.5e-7 W/S of noise).
+    S, M = plant.S, plant.M
+
+    def P_inv(x, Gv, inv, on_=on):
+        tot = 0.0
+        for s_ in range(S):
+            e = inv * S * M + s_ * M
+            tot += O.module_current(x[pos[e]], 9e-3 * Gv[e], 1e-10, 1.5, 0.3, rsh[e])[0]
+        return x[buses[inv]] * tot
+
     rng = np.random.default_rng(0)
-    per_inv = plant.S * plant.M
-    for m in rng.choice(plant.nmod, 6, replace=False):
-        inv = m // per_inv
+    for m in rng.choice(plant.nmod, 5, replace=False):
+        inv = m // (S * M)
 
-        def P_of(h):
-            Gh = G[:, k].copy()
-            Gh[m] += h
-            x, _, _ = on.solve(9e-3 * Gh, vo, ladder=False, tol=1e-13)
-            return _inverter_power(plant, x, inv)
+        def fdc(h):
+            vals = []
+            for sgn in (1, -1):
+                Gh = G[:, k].copy()
+                Gh[m] += sgn * h
+                x, _, _ = on.solve(9e-3 * Gh, vo, ladder=False, tol=1e-13)
+                vals.append(P_inv(x, Gh, inv))
+            return (vals[0] - vals[1]) / (2 * h)
 
-        h = 8.0
-        d1 = (P_of(h) - P_of(-h)) / (2 * h)
-        d2 = (P_of(h / 2) - P_of(-h / 2)) / h
-        fd = (4 * d2 - d1) / 3
-        assert dG[m, k] == pytest.approx(fd, rel=1e-6, abs=1e-7), m
+        fd = (4 * fdc(0.05) - fdc(0.1)) / 3
+        assert abs(dG[m, k] - fd) <= 1e-6 * abs(fd) + 1e-8, (m, dG[m, k], fd)
+    for c in (0, plant.ncab // 2):
+        inv = c // S
+
+        def Pc(gg):
+            g2 = gc.copy()
+            g2[c] = gg
+            on2 = O.PVNewton(plant.n, pos, neg, rsh, a, b, g2, buses, *plant_perm(plant))
+            x, _, _ = on2.solve(9e-3 * G[:, k], vo, ladder=False, tol=1e-13)
+            return P_inv(x, G[:, k], inv)
+
+        fd2 = (Pc(gc[c] + 0.02) - Pc(gc[c] - 0.02)) / 0.04
+        fd1 = (Pc(gc[c] + 0.01) - Pc(gc[c] - 0.01)) / 0.02
+        fd = (4 * fd1 - fd2) / 3
+        assert abs(dgc[c, k] - fd) <= 1e-6 * abs(fd) + 3e-8, (c, dgc[c, k], fd)
+
+
+def plant_perm(plant):
+    rp, ci = plant.pattern()
+    return pv.reorder(plant.n, rp, ci)
 
 
 def test_device_stamping_matches_oracle_assembly(cfg0, dev):

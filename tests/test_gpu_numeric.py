@@ -50,7 +50,8 @@ def _random_case(seed, n=None):
 
 
 SCHEDS = [(pv.SCHED_LEVEL, 1), (pv.SCHED_LEVEL, 64), (pv.SCHED_LEVEL, 16),
-          (pv.SCHED_COMPONENT, 1), (pv.SCHED_COMPONENT, 8), (pv.SCHED_COMPONENT, 64)]
+          (pv.SCHED_COMPONENT, 1), (pv.SCHED_COMPONENT, 8), (pv.SCHED_COMPONENT, 64),
+          (pv.SCHED_HYBRID, 1), (pv.SCHED_HYBRID, 4), (pv.SCHED_HYBRID, 64)]
 
 
 @pytest.mark.parametrize("seed", range(4))
@@ -112,17 +113,25 @@ def test_solve_transpose_bitexact(dev, sched):
     assert np.array_equal(dy.cpu().numpy(), want)
 
 
+PLANT_SCHEDS = [(pv.SCHED_AUTO, 0), (pv.SCHED_LEVEL, 0), (pv.SCHED_COMPONENT, 0),
+                (pv.SCHED_HYBRID, 0)]
+
+
 @pytest.mark.parametrize("cfg", [(4, 10, 20), (40, 24, 28)])
 @pytest.mark.parametrize("B", [1, 64, 130])
-def test_plant_refactor_solve(dev, cfg, B):
+@pytest.mark.parametrize("sched", PLANT_SCHEDS)
+def test_plant_refactor_solve(dev, cfg, B, sched):
     """Config plants (cfg0 4x10x20, cfg1 40x24x28) in the reference pipeline's
-    AMD ordering when oracle/_ref is built (natural order otherwise)."""
+    AMD ordering when oracle/_ref is built (natural order otherwise), under
+    every schedule (the DC bus chains mix register runs with generic columns;
+    HYBRID merges the narrow bus/substitution levels into one launch)."""
     I, S, M = cfg
     rng = np.random.default_rng(I * 100 + B)
     n, rp0, ci0 = plant_pattern(I, S, M)[0], *csr_from_sets(plant_pattern(I, S, M)[1])
     P = O.amd_order(n, rp0, ci0) if O.ref() is not None else np.arange(n, dtype=np.int32)
     rp, ci, _ = O.permute_sym(n, rp0, ci0, np.zeros(len(ci0)), P)
     plan = pv.Plan(n, rp, ci)
+    plan.set_schedule(*sched)
     s = O.csr_csc(n, rp, ci)
     Pinv = O.invert_permutation(P)
     _, _, _, V, bv = plant_values(rng, I, S, M, B)
