@@ -103,6 +103,8 @@ enum {
   PVGLU_INFO_NCOMP,
   PVGLU_INFO_FAST_LU_CHAINS, /* refactor chains on the register-forwarding path */
   PVGLU_INFO_FAST_U_CHAINS,  /* backward-substitution chains on that path */
+  PVGLU_INFO_FAST_LU_SEGMENTS, /* strided record segments, refactor */
+  PVGLU_INFO_FAST_U_SEGMENTS,  /* strided record segments, backward */
   PVGLU_INFO_COUNT
 };
 int pvglu_plan_info(const pvglu_plan *plan, int what);

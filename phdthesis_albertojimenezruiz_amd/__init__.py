@@ -27,7 +27,7 @@ SCHED_AUTO, SCHED_LEVEL, SCHED_COMPONENT, SCHED_HYBRID = 0, 1, 2, 3
 
 INFO = dict(n=0, nnz=1, nlev_lu=2, nchain_lu=3, nlev_l=4, nchain_l=5, nlev_u=6,
             nchain_u=7, nlev_lt=8, nchain_lt=9, nops=10, fused_lsolve=11, ncomp=12,
-            fast_lu_chains=13, fast_u_chains=14)
+            fast_lu_chains=13, fast_u_chains=14, fast_lu_segments=15, fast_u_segments=16)
 ARRAYS = dict(csr_rowptr=0, csr_colidx=1, csc_colptr=2, csc_rowidx=3, csr2csc=4,
               csc2csr=5, csr_diag=6, csc_diag=7, lu_start_levels=8, lu_start_chains=9,
               lu_chains=10, l_start_levels=11, l_start_chains=12, l_chains=13,

@@ -57,8 +57,9 @@ struct DevicePlan {
       *lrow_slot = nullptr, *lrow_col = nullptr, *urow_ptr = nullptr,
       *urow_slot = nullptr, *urow_col = nullptr;
   DevLevels lu, l, u, lt;
-  int *lu_rec = nullptr, *lu_run = nullptr, *lu_fast = nullptr,
-      *u_rec = nullptr, *u_run = nullptr, *u_fast = nullptr;
+  int *lu_rec = nullptr, *lu_seg = nullptr, *lu_run = nullptr,
+      *lu_fast = nullptr, *u_rec = nullptr, *u_seg = nullptr, *u_run = nullptr,
+      *u_fast = nullptr;
   int *status = nullptr; // [code, column, instance, pad]
   std::vector<void *> allocs;
 };
@@ -80,9 +81,11 @@ struct Pat {
   const int *__restrict__ urow_slot;
   const int *__restrict__ urow_col;
   const int *__restrict__ lu_rec;
+  const int *__restrict__ lu_seg;
   const int *__restrict__ lu_run;
   const int *__restrict__ lu_fast;
   const int *__restrict__ u_rec;
+  const int *__restrict__ u_seg;
   const int *__restrict__ u_run;
   const int *__restrict__ u_fast;
   int *status;
@@ -101,8 +104,9 @@ struct Lev {
 static Pat pat_of(const DevicePlan *d) {
   return Pat{d->cp,       d->ri,        d->op_ptr,    d->op_l,     d->op_u,
              d->diag_slot, d->lrow_ptr, d->lrow_slot, d->lrow_col, d->urow_ptr,
-             d->urow_slot, d->urow_col, d->lu_rec,  d->lu_run,    d->lu_fast,
-             d->u_rec,     d->u_run,    d->u_fast, d->status};
+             d->urow_slot, d->urow_col, d->lu_rec,  d->lu_seg,    d->lu_run,
+             d->lu_fast,   d->u_rec,    d->u_seg,  d->u_run,     d->u_fast,
+             d->status};
 }
 static Lev lev_of(const DevLevels &L) {
   return Lev{L.level_chain_ptr, L.chain_col_ptr, L.cols,
@@ -260,10 +264,12 @@ __device__ __forceinline__ void ltsolve_col(const Pat &P, int j,
 
 
 // ------------------------------------------------ register-forwarding chains
-// Chains whose columns have <= 4 entries run from the 16-int records built by
+// Runs of columns with <= 4 entries execute from the 16-int records built by
 // compile_fast_programs: the column's values live in registers, operands that
-// the previous column of the chain produced are forwarded from registers, and
-// the next column's A and b values are loaded while the current one computes.
+// the previous column of the run produced are forwarded from registers, and
+// the A and b values two columns ahead are loaded while the current column
+// computes.  A strided segment shares one record: its columns' slots and rows
+// are computed, so inside a segment no pattern data is loaded at all.
 // Each entry still sees the generic path's operation sequence (bit-identical).
 
 __device__ __forceinline__ double sel4(double v0, double v1, double v2,
@@ -271,53 +277,71 @@ __device__ __forceinline__ double sel4(double v0, double v1, double v2,
   return i == 0 ? v0 : (i == 1 ? v1 : (i == 2 ? v2 : v3));
 }
 
-// A prefetched refactor column: record header and its A and b values.
-struct FastCol {
-  const int *r;
-  int slot0, col, meta;
+template <int W, int N>
+__device__ __forceinline__ void load_record(const int *r, int (&f)[N]) {
+  const int4 *r4 = reinterpret_cast<const int4 *>(r);
+#pragma unroll
+  for (int k = 0; k < N / 4; ++k) {
+    const int4 v = r4[k];
+    f[4 * k] = uniform<W>(v.x);
+    f[4 * k + 1] = uniform<W>(v.y);
+    f[4 * k + 2] = uniform<W>(v.z);
+    f[4 * k + 3] = uniform<W>(v.w);
+  }
+}
+
+// A and b values of one refactor column.
+struct ColVals {
   double a0, a1, a2, a3, bb;
 };
 
-template <int W>
-__device__ __forceinline__ void fast_lu_load(FastCol &c, const int *r,
-                                             const double *A, const double *b,
-                                             size_t B, int inst) {
-  c.r = r;
-  c.slot0 = uniform<W>(r[0]);
-  c.col = uniform<W>(r[1]);
-  c.meta = uniform<W>(r[2]);
-  const int ne = c.meta & 7;
-  c.a0 = A[(size_t)c.slot0 * B + inst];
-  c.a1 = ne > 1 ? A[(size_t)(c.slot0 + 1) * B + inst] : 0.0;
-  c.a2 = ne > 2 ? A[(size_t)(c.slot0 + 2) * B + inst] : 0.0;
-  c.a3 = ne > 3 ? A[(size_t)(c.slot0 + 3) * B + inst] : 0.0;
-  c.bb = b[(size_t)c.col * B + inst];
+__device__ __forceinline__ void load_col(ColVals &c, int slot0, int col, int ne,
+                                         const double *A, const double *b,
+                                         size_t B, int inst) {
+  c.a0 = A[(size_t)slot0 * B + inst];
+  c.a1 = ne > 1 ? A[(size_t)(slot0 + 1) * B + inst] : 0.0;
+  c.a2 = ne > 2 ? A[(size_t)(slot0 + 2) * B + inst] : 0.0;
+  c.a3 = ne > 3 ? A[(size_t)(slot0 + 3) * B + inst] : 0.0;
+  c.bb = b[(size_t)col * B + inst];
 }
 
-template <int W>
-__device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
-                                              const double *A, double *LU,
-                                              const double *b, double *x,
-                                              size_t B, int inst) {
-  double p0 = 0, p1 = 0, p2 = 0, p3 = 0, yprev = 0;
-  // two columns in flight ahead of the one being computed
-  FastCol c0, c1, c2;
-  const int *rec = P.lu_rec;
-  fast_lu_load<W>(c0, rec + (size_t)t0 * 16, A, b, B, inst);
-  if (t0 + 1 < t1)
-    fast_lu_load<W>(c1, rec + (size_t)(t0 + 1) * 16, A, b, B, inst);
-  for (int t = t0; t < t1; ++t) {
-    if (t + 2 < t1)
-      fast_lu_load<W>(c2, rec + (size_t)(t + 2) * 16, A, b, B, inst);
-    const int *r = c0.r;
-    const int slot0 = c0.slot0, col = c0.col, meta = c0.meta, ne = meta & 7;
-    double v0 = c0.a0, v1 = c0.a1, v2 = c0.a2, v3 = c0.a3;
-    const int nops = (meta >> 6) & 7;
-    for (int k = 0; k < nops; ++k) {
-      const int lsrc = uniform<W>(r[4 + 2 * k]);
-      const int tu = uniform<W>(r[5 + 2 * k]);
-      const double l =
-          lsrc >= 0 ? LU[(size_t)lsrc * B + inst] : sel4(p0, p1, p2, p3, -1 - lsrc);
+// A segment whose operands are all forwarded from the previous column runs
+// as a three-stage pipeline with a fixed sequence of memory operations per
+// column (the loads of column i+2 are issued before column i computes, stores
+// are unconditional in NE, the loop is unrolled three times so no register
+// copies of in-flight loads are needed): the compiler's counter-based waits
+// then wait for the column being computed only, not for the prefetches.
+template <int NE>
+__device__ __forceinline__ void pure_load_col(ColVals &c, int slot0, int col,
+                                              const double *A,
+                                              const double *b, size_t B,
+                                              int inst) {
+  c.a0 = A[(size_t)slot0 * B + inst];
+  if constexpr (NE > 1)
+    c.a1 = A[(size_t)(slot0 + 1) * B + inst];
+  if constexpr (NE > 2)
+    c.a2 = A[(size_t)(slot0 + 2) * B + inst];
+  if constexpr (NE > 3)
+    c.a3 = A[(size_t)(slot0 + 3) * B + inst];
+  c.bb = b[(size_t)col * B + inst];
+}
+
+template <int NE>
+__device__ __forceinline__ void pure_lu_col(const ColVals &c, int slot0,
+                                            int col, const int (&f)[16],
+                                            double *LU, double *x, size_t B,
+                                            int inst, double &p0, double &p1,
+                                            double &p2, double &p3,
+                                            double &yprev, int &badcol) {
+  const int meta = f[2];
+  const int d = (meta >> 3) & 7, nops = (meta >> 6) & 7, nlr = (meta >> 9) & 3;
+  double v0 = c.a0, v1 = NE > 1 ? c.a1 : 0.0, v2 = NE > 2 ? c.a2 : 0.0,
+         v3 = NE > 3 ? c.a3 : 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (k < nops) {
+      const int tu = f[5 + 2 * k];
+      const double l = sel4(p0, p1, p2, p3, -1 - f[4 + 2 * k]);
       const double prod = l * sel4(v0, v1, v2, v3, tu >> 2);
       switch (tu & 3) {
       case 0: v0 = v0 - prod; break;
@@ -326,90 +350,270 @@ __device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
       default: v3 = v3 - prod; break;
       }
     }
-    const int d = (meta >> 3) & 7;
-    const double piv = sel4(v0, v1, v2, v3, d);
-    if (fabs(piv) < 1e-8)
-      record_status(P.status, PVGLU_E_SINGULAR, col, inst);
-    if (d < 1 && ne > 1)
-      v1 = v1 / piv;
-    if (d < 2 && ne > 2)
-      v2 = v2 / piv;
-    if (d < 3 && ne > 3)
-      v3 = v3 / piv;
-    LU[(size_t)slot0 * B + inst] = v0;
-    if (ne > 1)
-      LU[(size_t)(slot0 + 1) * B + inst] = v1;
-    if (ne > 2)
-      LU[(size_t)(slot0 + 2) * B + inst] = v2;
-    if (ne > 3)
-      LU[(size_t)(slot0 + 3) * B + inst] = v3;
-    // forward substitution of row col
-    const int nlr = (meta >> 9) & 3;
-    double y = c0.bb;
-    for (int k = 0; k < nlr; ++k) {
-      const int lsrc = uniform<W>(r[12 + 2 * k]);
-      const int ysrc = uniform<W>(r[13 + 2 * k]);
-      const double l =
-          lsrc >= 0 ? LU[(size_t)lsrc * B + inst] : sel4(p0, p1, p2, p3, -1 - lsrc);
-      const double yj = ysrc >= 0 ? x[(size_t)ysrc * B + inst] : yprev;
-      y = y - l * yj;
-    }
-    x[(size_t)col * B + inst] = y;
-    yprev = y;
-    p0 = v0;
-    p1 = v1;
-    p2 = v2;
-    p3 = v3;
-    c0 = c1;
-    c1 = c2;
   }
+  const double piv = sel4(v0, v1, v2, v3, d);
+  if (fabs(piv) < 1e-8 && badcol < 0)
+    badcol = col;
+  if (NE > 1 && d < 1)
+    v1 = v1 / piv;
+  if (NE > 2 && d < 2)
+    v2 = v2 / piv;
+  if (NE > 3 && d < 3)
+    v3 = v3 / piv;
+  LU[(size_t)slot0 * B + inst] = v0;
+  if constexpr (NE > 1)
+    LU[(size_t)(slot0 + 1) * B + inst] = v1;
+  if constexpr (NE > 2)
+    LU[(size_t)(slot0 + 2) * B + inst] = v2;
+  if constexpr (NE > 3)
+    LU[(size_t)(slot0 + 3) * B + inst] = v3;
+  double y = c.bb;
+  if (nlr > 0)
+    y = y - sel4(p0, p1, p2, p3, -1 - f[12]) * yprev;
+  x[(size_t)col * B + inst] = y;
+  yprev = y;
+  p0 = v0;
+  p1 = v1;
+  p2 = v2;
+  p3 = v3;
+}
+
+template <int NE>
+__device__ __forceinline__ void pure_lu_segment(
+    const int (&f)[16], int len, int dslot, int dcol, const double *A,
+    double *LU, const double *b, double *x, size_t B, int inst, double &p0,
+    double &p1, double &p2, double &p3, double &yprev, int &badcol) {
+  ColVals s0, s1, s2;
+  auto ld = [&](ColVals &c, int i) {
+    i = min(i, len - 1);
+    pure_load_col<NE>(c, f[0] + i * dslot, f[1] + i * dcol, A, b, B, inst);
+  };
+  auto run = [&](const ColVals &c, int i) {
+    pure_lu_col<NE>(c, f[0] + i * dslot, f[1] + i * dcol, f, LU, x, B, inst,
+                    p0, p1, p2, p3, yprev, badcol);
+  };
+  ld(s0, 0);
+  ld(s1, 1);
+  for (int i = 0;;) {
+    ld(s2, i + 2);
+    run(s0, i);
+    if (++i >= len)
+      break;
+    ld(s0, i + 2);
+    run(s1, i);
+    if (++i >= len)
+      break;
+    ld(s1, i + 2);
+    run(s2, i);
+    if (++i >= len)
+      break;
+  }
+}
+
+// Backward-substitution values of one row: y, U(i,i) and the two U terms.
+struct RowVals {
+  double y, dv, u0, u1;
+};
+
+template <int NT>
+__device__ __forceinline__ void pure_u_segment(const int (&f)[8], int len,
+                                               int drow, int ddiag,
+                                               const double *LU, double *x,
+                                               size_t B, int inst,
+                                               double &xprev) {
+  RowVals s0, s1, s2;
+  auto ld = [&](RowVals &r, int i) {
+    i = min(i, len - 1);
+    const int row = f[0] + i * drow, dslot = f[1] + i * ddiag;
+    r.y = x[(size_t)row * B + inst];
+    r.dv = LU[(size_t)dslot * B + inst];
+    if constexpr (NT > 0)
+      r.u0 = LU[(size_t)(dslot + f[4]) * B + inst];
+  };
+  auto run = [&](const RowVals &r, int i) {
+    double s = r.y;
+    if constexpr (NT > 0)
+      s = s - r.u0 * xprev;
+    const double xi = s / r.dv;
+    x[(size_t)(f[0] + i * drow) * B + inst] = xi;
+    xprev = xi;
+  };
+  ld(s0, 0);
+  ld(s1, 1);
+  for (int i = 0;;) {
+    ld(s2, i + 2);
+    run(s0, i);
+    if (++i >= len)
+      break;
+    ld(s0, i + 2);
+    run(s1, i);
+    if (++i >= len)
+      break;
+    ld(s1, i + 2);
+    run(s2, i);
+    if (++i >= len)
+      break;
+  }
+}
+
+template <int W>
+__device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
+                                              const double *A, double *LU,
+                                              const double *b, double *x,
+                                              size_t B, int inst) {
+  double p0 = 0, p1 = 0, p2 = 0, p3 = 0, yprev = 0;
+  for (int t = t0; t < t1;) {
+    const int *sg = P.lu_seg + (size_t)t * 4;
+    const int len = uniform<W>(sg[0]);
+    const int dslot = uniform<W>(sg[1]);
+    const int dcol = uniform<W>(sg[2]);
+    int f[16];
+    load_record<W>(P.lu_rec + (size_t)t * 16, f);
+    const int meta = f[2];
+    const int ne = meta & 7, d = (meta >> 3) & 7, nops = (meta >> 6) & 7,
+              nlr = (meta >> 9) & 3;
+    if (uniform<W>(sg[3])) {
+      int badcol = -1;
+      switch (ne) {
+      case 1:
+        pure_lu_segment<1>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
+                           p2, p3, yprev, badcol);
+        break;
+      case 2:
+        pure_lu_segment<2>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
+                           p2, p3, yprev, badcol);
+        break;
+      case 3:
+        pure_lu_segment<3>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
+                           p2, p3, yprev, badcol);
+        break;
+      default:
+        pure_lu_segment<4>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
+                           p2, p3, yprev, badcol);
+        break;
+      }
+      if (badcol >= 0)
+        record_status(P.status, PVGLU_E_SINGULAR, badcol, inst);
+      t += len;
+      continue;
+    }
+    ColVals c0, c1, c2;
+    load_col(c0, f[0], f[1], ne, A, b, B, inst);
+    if (len > 1)
+      load_col(c1, f[0] + dslot, f[1] + dcol, ne, A, b, B, inst);
+    for (int i = 0; i < len; ++i) {
+      if (i + 2 < len)
+        load_col(c2, f[0] + (i + 2) * dslot, f[1] + (i + 2) * dcol, ne, A, b,
+                 B, inst);
+      const int slot0 = f[0] + i * dslot, col = f[1] + i * dcol;
+      double v0 = c0.a0, v1 = c0.a1, v2 = c0.a2, v3 = c0.a3;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (k < nops) {
+          const int lsrc = f[4 + 2 * k], tu = f[5 + 2 * k];
+          const double l = lsrc >= 0 ? LU[(size_t)lsrc * B + inst]
+                                     : sel4(p0, p1, p2, p3, -1 - lsrc);
+          const double prod = l * sel4(v0, v1, v2, v3, tu >> 2);
+          switch (tu & 3) {
+          case 0: v0 = v0 - prod; break;
+          case 1: v1 = v1 - prod; break;
+          case 2: v2 = v2 - prod; break;
+          default: v3 = v3 - prod; break;
+          }
+        }
+      }
+      const double piv = sel4(v0, v1, v2, v3, d);
+      if (fabs(piv) < 1e-8)
+        record_status(P.status, PVGLU_E_SINGULAR, col, inst);
+      if (d < 1 && ne > 1)
+        v1 = v1 / piv;
+      if (d < 2 && ne > 2)
+        v2 = v2 / piv;
+      if (d < 3 && ne > 3)
+        v3 = v3 / piv;
+      LU[(size_t)slot0 * B + inst] = v0;
+      if (ne > 1)
+        LU[(size_t)(slot0 + 1) * B + inst] = v1;
+      if (ne > 2)
+        LU[(size_t)(slot0 + 2) * B + inst] = v2;
+      if (ne > 3)
+        LU[(size_t)(slot0 + 3) * B + inst] = v3;
+      // forward substitution of row col
+      double y = c0.bb;
+#pragma unroll
+      for (int k = 0; k < 2; ++k) {
+        if (k < nlr) {
+          const int lsrc = f[12 + 2 * k], ysrc = f[13 + 2 * k];
+          const double l = lsrc >= 0 ? LU[(size_t)lsrc * B + inst]
+                                     : sel4(p0, p1, p2, p3, -1 - lsrc);
+          const double yj = ysrc >= 0 ? x[(size_t)ysrc * B + inst] : yprev;
+          y = y - l * yj;
+        }
+      }
+      x[(size_t)col * B + inst] = y;
+      yprev = y;
+      p0 = v0;
+      p1 = v1;
+      p2 = v2;
+      p3 = v3;
+      c0 = c1;
+      c1 = c2;
+    }
+    t += len;
+  }
+}
+
+__device__ __forceinline__ void load_row(RowVals &v, int row, int dslot, int nt,
+                                         int rel0, int rel1, const double *LU,
+                                         const double *x, size_t B, int inst) {
+  v.y = x[(size_t)row * B + inst];
+  v.dv = LU[(size_t)dslot * B + inst];
+  v.u0 = nt > 0 ? LU[(size_t)(dslot + rel0) * B + inst] : 0.0;
+  v.u1 = nt > 1 ? LU[(size_t)(dslot + rel1) * B + inst] : 0.0;
 }
 
 template <int W>
 __device__ __forceinline__ void fast_u_chain(const Pat &P, int t0, int t1,
                                              const double *LU, double *x,
                                              size_t B, int inst) {
-  const int *r = P.u_rec + (size_t)t0 * 8;
-  int row = uniform<W>(r[0]), dslot = uniform<W>(r[1]), nt = uniform<W>(r[2]);
-  double y = x[(size_t)row * B + inst];
-  double dv = LU[(size_t)dslot * B + inst];
-  double u0 = nt > 0 ? LU[(size_t)uniform<W>(r[4]) * B + inst] : 0.0;
-  double u1 = nt > 1 ? LU[(size_t)uniform<W>(r[6]) * B + inst] : 0.0;
   double xprev = 0;
-  for (int t = t0; t < t1; ++t) {
-    const int *nr = r + 8;
-    int nrow = 0, ndslot = 0, nnt = 0;
-    double ny = 0, ndv = 0, nu0 = 0, nu1 = 0;
-    if (t + 1 < t1) {
-      nrow = uniform<W>(nr[0]);
-      ndslot = uniform<W>(nr[1]);
-      nnt = uniform<W>(nr[2]);
-      ny = x[(size_t)nrow * B + inst];
-      ndv = LU[(size_t)ndslot * B + inst];
-      if (nnt > 0)
-        nu0 = LU[(size_t)uniform<W>(nr[4]) * B + inst];
-      if (nnt > 1)
-        nu1 = LU[(size_t)uniform<W>(nr[6]) * B + inst];
+  for (int t = t0; t < t1;) {
+    const int *sg = P.u_seg + (size_t)t * 4;
+    const int len = uniform<W>(sg[0]);
+    const int drow = uniform<W>(sg[1]);
+    const int ddiag = uniform<W>(sg[2]);
+    int f[8];
+    load_record<W>(P.u_rec + (size_t)t * 8, f);
+    const int nt = f[2];
+    if (uniform<W>(sg[3])) {
+      if (nt == 0)
+        pure_u_segment<0>(f, len, drow, ddiag, LU, x, B, inst, xprev);
+      else
+        pure_u_segment<1>(f, len, drow, ddiag, LU, x, B, inst, xprev);
+      t += len;
+      continue;
     }
-    double s = y;
-    if (nt > 0) {
-      const int xs = uniform<W>(r[5]);
-      s = s - u0 * (xs >= 0 ? x[(size_t)xs * B + inst] : xprev);
+    RowVals r0, r1, r2;
+    load_row(r0, f[0], f[1], nt, f[4], f[6], LU, x, B, inst);
+    if (len > 1)
+      load_row(r1, f[0] + drow, f[1] + ddiag, nt, f[4], f[6], LU, x, B, inst);
+    for (int i = 0; i < len; ++i) {
+      if (i + 2 < len)
+        load_row(r2, f[0] + (i + 2) * drow, f[1] + (i + 2) * ddiag, nt, f[4],
+                 f[6], LU, x, B, inst);
+      const int row = f[0] + i * drow;
+      double s = r0.y;
+      if (nt > 0)
+        s = s - r0.u0 * (f[5] >= 0 ? x[(size_t)f[5] * B + inst] : xprev);
+      if (nt > 1)
+        s = s - r0.u1 * (f[7] >= 0 ? x[(size_t)f[7] * B + inst] : xprev);
+      const double xi = s / r0.dv;
+      x[(size_t)row * B + inst] = xi;
+      xprev = xi;
+      r0 = r1;
+      r1 = r2;
     }
-    if (nt > 1) {
-      const int xs = uniform<W>(r[7]);
-      s = s - u1 * (xs >= 0 ? x[(size_t)xs * B + inst] : xprev);
-    }
-    const double xi = s / dv;
-    x[(size_t)row * B + inst] = xi;
-    xprev = xi;
-    row = nrow;
-    nt = nnt;
-    y = ny;
-    dv = ndv;
-    u0 = nu0;
-    u1 = nu1;
-    r = nr;
+    t += len;
   }
 }
 
@@ -799,6 +1003,8 @@ extern "C" int pvglu_plan_upload(pvglu_plan *P) {
       (st = dev_upload(d, &d->lu_rec, P->fast_lu_rec)) ||
       (st = dev_upload(d, &d->lu_fast, P->fast_lu_chain)) ||
       (st = dev_upload(d, &d->lu_run, P->fast_lu_run)) ||
+      (st = dev_upload(d, &d->lu_seg, P->fast_lu_seg)) ||
+      (st = dev_upload(d, &d->u_seg, P->fast_u_seg)) ||
       (st = dev_upload(d, &d->u_run, P->fast_u_run)) ||
       (st = dev_upload(d, &d->u_rec, P->fast_u_rec)) ||
       (st = dev_upload(d, &d->u_fast, P->fast_u_chain)) ||

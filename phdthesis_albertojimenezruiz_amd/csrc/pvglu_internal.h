@@ -78,9 +78,9 @@ struct pvglu_plan {
   // record per refactor column and one 8-int record per backward-substitution
   // row, in kernel order, the end of the register run holding each position
   // (0: generic column) and a per-chain whole-chain flag.
-  std::vector<int> fast_lu_rec, fast_lu_run, fast_lu_chain;
-  std::vector<int> fast_u_rec, fast_u_run, fast_u_chain;
-  int fast_lu_chains = 0, fast_u_chains = 0;
+  std::vector<int> fast_lu_rec, fast_lu_run, fast_lu_chain, fast_lu_seg;
+  std::vector<int> fast_u_rec, fast_u_run, fast_u_chain, fast_u_seg;
+  int fast_lu_chains = 0, fast_u_chains = 0, fast_lu_segs = 0, fast_u_segs = 0;
   uint64_t hash = 0;
 
   // schedule choice

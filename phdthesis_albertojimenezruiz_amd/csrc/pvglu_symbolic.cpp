@@ -240,7 +240,15 @@ void build_kernel_levels(const LevelSet &ls, const std::vector<int> &comp_of,
 //   [13+2k] y operand: >=0 row, -1 the previous column's y (a register)
 // Backward record (8 ints) for the row at kernel position t of the U levels:
 //   [0] row  [1] diagonal slot  [2] nterms  [3] unused
-//   [4+2k] U slot of term k  [5+2k] x operand: >=0 row, -1 previous row's x
+//   [4+2k] U slot of term k minus the diagonal slot
+//   [5+2k] x operand: >=0 row, -1 previous row's x
+// Strided segments: consecutive positions of a run whose records agree in
+// every field but [0] and [1], which advance by constant steps (a series
+// string: the same tridiagonal elimination step column after column), share
+// one record; seg[4t..4t+2] = (length, step of [0], step of [1]) at each
+// segment start, so the kernel loads one record per segment and computes the
+// slots and rows of the following columns; seg[4t+3] flags a segment whose
+// operands all come from the previous column's registers.
 // run[t] = end position of the register run that contains t (0 if t does not
 // qualify); chain flag = 1 when one run covers the whole chain.
 static constexpr int kFastMaxE = 4, kFastMaxOps = 4, kFastMaxLr = 2;
@@ -284,7 +292,7 @@ static void encode_u_record(const pvglu_plan *P, int i, int prev, int *r) {
   r[2] = nt;
   for (int k = 0; k < nt; k++) {
     const int j = P->urow_col[P->urow_ptr[i] + k];
-    r[4 + 2 * k] = P->urow_slot[P->urow_ptr[i] + k];
+    r[4 + 2 * k] = P->urow_slot[P->urow_ptr[i] + k] - P->diag_slot[i];
     r[5 + 2 * k] = (j == prev) ? -1 : j;
   }
 }
@@ -327,6 +335,47 @@ static int compile_runs(const KernelLevels &K, int n, int rec_ints, Q qualifies,
   return whole;
 }
 
+template <class F>
+static void compile_segments(const std::vector<int> &rec, int rec_ints,
+                             const std::vector<int> &run, int n,
+                             std::vector<int> &seg, F forwarded) {
+  seg.assign((size_t)std::max(n, 1) * 4, 0);
+  auto same = [&](int a, int b) {
+    for (int k = 2; k < rec_ints; k++)
+      if (rec[(size_t)a * rec_ints + k] != rec[(size_t)b * rec_ints + k])
+        return false;
+    return true;
+  };
+  auto step = [&](int a, int k) {
+    return rec[(size_t)(a + 1) * rec_ints + k] - rec[(size_t)a * rec_ints + k];
+  };
+  for (int t = 0; t < n;) {
+    const int e = run[t];
+    if (e <= t) {
+      ++t;
+      continue;
+    }
+    // t starts a run [t, e)
+    for (int u = t; u < e;) {
+      int len = 1, d0 = 0, d1 = 0;
+      if (u + 1 < e && same(u, u + 1)) {
+        d0 = step(u, 0);
+        d1 = step(u, 1);
+        len = 2;
+        while (u + len < e && same(u + len - 1, u + len) &&
+               step(u + len - 1, 0) == d0 && step(u + len - 1, 1) == d1)
+          ++len;
+      }
+      seg[(size_t)u * 4] = len;
+      seg[(size_t)u * 4 + 1] = d0;
+      seg[(size_t)u * 4 + 2] = d1;
+      seg[(size_t)u * 4 + 3] = forwarded(&rec[(size_t)u * rec_ints]) ? 1 : 0;
+      u += len;
+    }
+    t = e;
+  }
+}
+
 void compile_fast_programs(pvglu_plan *P) {
   if (P->fuse_lsolve) {
     P->fast_lu_chains = compile_runs(
@@ -348,6 +397,28 @@ void compile_fast_programs(pvglu_plan *P) {
       },
       [&](int i, int prev, int *r) { encode_u_record(P, i, prev, r); },
       P->fast_u_rec, P->fast_u_run, P->fast_u_chain);
+  // seg[4t+3] = 1 when every operand of the record comes from the previous
+  // column's registers (no pattern-addressed loads inside the segment): the
+  // kernel then runs the segment as a static three-stage load pipeline.
+  compile_segments(P->fast_lu_rec, 16, P->fast_lu_run, P->n, P->fast_lu_seg,
+                   [](const int *r) {
+                     const int nops = (r[2] >> 6) & 7, nlr = (r[2] >> 9) & 3;
+                     for (int k = 0; k < nops; k++)
+                       if (r[4 + 2 * k] >= 0)
+                         return false;
+                     if (nlr > 1)
+                       return false;
+                     return nlr == 0 || (r[12] < 0 && r[13] == -1);
+                   });
+  compile_segments(P->fast_u_rec, 8, P->fast_u_run, P->n, P->fast_u_seg,
+                   [](const int *r) {
+                     return r[2] == 0 || (r[2] == 1 && r[5] == -1);
+                   });
+  P->fast_lu_segs = P->fast_u_segs = 0;
+  for (int t = 0; t < P->n; t++) {
+    P->fast_lu_segs += P->fast_lu_seg[(size_t)t * 4] > 0;
+    P->fast_u_segs += P->fast_u_seg[(size_t)t * 4] > 0;
+  }
 }
 
 } // namespace pvglu
@@ -602,6 +673,8 @@ extern "C" int pvglu_plan_info(const pvglu_plan *P, int what) {
   case PVGLU_INFO_NCOMP: return P->ncomp;
   case PVGLU_INFO_FAST_LU_CHAINS: return P->fast_lu_chains;
   case PVGLU_INFO_FAST_U_CHAINS: return P->fast_u_chains;
+  case PVGLU_INFO_FAST_LU_SEGMENTS: return P->fast_lu_segs;
+  case PVGLU_INFO_FAST_U_SEGMENTS: return P->fast_u_segs;
   default: return fail(PVGLU_E_ARG, "plan_info: unknown field %d", what);
   }
 }
