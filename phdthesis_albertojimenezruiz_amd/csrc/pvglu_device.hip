@@ -26,6 +26,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 using namespace pvglu;
@@ -59,7 +60,8 @@ struct DevicePlan {
   DevLevels lu, l, u, lt;
   int *lu_rec = nullptr, *lu_seg = nullptr, *lu_run = nullptr,
       *lu_fast = nullptr, *u_rec = nullptr, *u_seg = nullptr, *u_run = nullptr,
-      *u_fast = nullptr;
+      *u_fast = nullptr, *l_rec = nullptr, *l_seg = nullptr, *l_run = nullptr,
+      *l_fast = nullptr;
   int *status = nullptr; // [code, column, instance, pad]
   std::vector<void *> allocs;
 };
@@ -88,6 +90,10 @@ struct Pat {
   const int *__restrict__ u_seg;
   const int *__restrict__ u_run;
   const int *__restrict__ u_fast;
+  const int *__restrict__ l_rec;
+  const int *__restrict__ l_seg;
+  const int *__restrict__ l_run;
+  const int *__restrict__ l_fast;
   int *status;
 };
 
@@ -106,7 +112,7 @@ static Pat pat_of(const DevicePlan *d) {
              d->diag_slot, d->lrow_ptr, d->lrow_slot, d->lrow_col, d->urow_ptr,
              d->urow_slot, d->urow_col, d->lu_rec,  d->lu_seg,    d->lu_run,
              d->lu_fast,   d->u_rec,    d->u_seg,  d->u_run,     d->u_fast,
-             d->status};
+             d->l_rec,     d->l_seg,    d->l_run,  d->l_fast,    d->status};
 }
 static Lev lev_of(const DevLevels &L) {
   return Lev{L.level_chain_ptr, L.chain_col_ptr, L.cols,
@@ -139,27 +145,64 @@ __device__ __forceinline__ void record_status(int *status, int code, int col,
 // of one per update.
 constexpr int kChunk = 8;
 
+// One chunk: CH operand pairs loaded unconditionally (indices past the end
+// repeat the last pair), then the updates applied in order.  GET(k) returns
+// the two operands of update k.
+template <int CH, class GET>
+__device__ __forceinline__ double chunk_sub(double v, int kb, int k1, GET get) {
+  double l[CH], r[CH];
+#pragma unroll
+  for (int j = 0; j < CH; ++j)
+    get(min(kb + j, k1 - 1), l[j], r[j]);
+#pragma unroll
+  for (int j = 0; j < CH; ++j)
+    if (kb + j < k1)
+      v = v - l[j] * r[j];
+  return v;
+}
+
+// v - sum_k l_k r_k over [k0, k1) in ascending k: full chunks of kChunk
+// independent loads, then one chunk sized to the remainder, so a sum costs
+// ceil(n / kChunk) memory round trips and issues at most 2 n + kChunk loads.
+template <class GET>
+__device__ __forceinline__ double dot_sub(double v, int k0, int k1, GET get) {
+  int kb = k0;
+  for (; k1 - kb >= kChunk; kb += kChunk)
+    v = chunk_sub<kChunk>(v, kb, k1, get);
+  const int rest = k1 - kb;
+  if (rest > kChunk / 2)
+    v = chunk_sub<kChunk>(v, kb, k1, get);
+  else if (rest > 2)
+    v = chunk_sub<kChunk / 2>(v, kb, k1, get);
+  else if (rest == 2)
+    v = chunk_sub<2>(v, kb, k1, get);
+  else if (rest == 1)
+    v = chunk_sub<1>(v, kb, k1, get);
+  return v;
+}
+
 template <int W>
 __device__ __forceinline__ double dot_sub_chunked(double v, const int *sl_arr,
                                                   const int *sr_arr,
                                                   const double *L,
                                                   const double *R, int k0,
                                                   int k1, size_t B, int inst) {
-  for (int kb = k0; kb < k1; kb += kChunk) {
-    const int cnt = min(kChunk, k1 - kb);
-    double l[kChunk], r[kChunk];
-#pragma unroll
-    for (int j = 0; j < kChunk; ++j)
-      if (j < cnt) {
-        l[j] = L[(size_t)uniform<W>(sl_arr[kb + j]) * B + inst];
-        r[j] = R[(size_t)uniform<W>(sr_arr[kb + j]) * B + inst];
-      }
-#pragma unroll
-    for (int j = 0; j < kChunk; ++j)
-      if (j < cnt)
-        v = v - l[j] * r[j];
-  }
-  return v;
+  return dot_sub(v, k0, k1, [&](int k, double &l, double &r) {
+    l = L[(size_t)uniform<W>(sl_arr[k]) * B + inst];
+    r = R[(size_t)uniform<W>(sr_arr[k]) * B + inst];
+  });
+}
+
+// Same over consecutive CSC slots q of LU against y at the rows ri[q].
+template <int W>
+__device__ __forceinline__ double dot_sub_csc(double v, const int *ri,
+                                              const double *LU,
+                                              const double *y, int q0, int q1,
+                                              size_t B, int inst) {
+  return dot_sub(v, q0, q1, [&](int q, double &l, double &r) {
+    l = LU[(size_t)q * B + inst];
+    r = y[(size_t)uniform<W>(ri[q]) * B + inst];
+  });
 }
 
 template <int W, bool FUSE>
@@ -171,15 +214,17 @@ __device__ __forceinline__ void refactor_column(const Pat &P, int c,
   const int q1 = uniform<W>(P.cp[c + 1]);
   const int qd = uniform<W>(P.diag[c]);
   double piv = 1.0;
-  for (int qb = q0; qb < q1; qb += kChunk) {
-    const int cnt = min(kChunk, q1 - qb);
-    double a[kChunk];
+  // the A values of CH entries are loaded together, then the entries are
+  // updated in slot order
+  auto entries = [&](auto ch, int qb) {
+    constexpr int CH = decltype(ch)::value;
+    const int cnt = min(CH, q1 - qb);
+    double a[CH];
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k)
-      if (k < cnt)
-        a[k] = A[(size_t)(qb + k) * B + inst];
+    for (int k = 0; k < CH; ++k)
+      a[k] = A[(size_t)min(qb + k, q1 - 1) * B + inst];
 #pragma unroll
-    for (int k = 0; k < kChunk; ++k) {
+    for (int k = 0; k < CH; ++k) {
       if (k >= cnt)
         break;
       const int q = qb + k;
@@ -195,6 +240,12 @@ __device__ __forceinline__ void refactor_column(const Pat &P, int c,
       }
       LU[(size_t)q * B + inst] = v;
     }
+  };
+  for (int qb = q0; qb < q1; qb += kChunk) {
+    if (q1 - qb > kChunk / 2)
+      entries(std::integral_constant<int, kChunk>(), qb);
+    else
+      entries(std::integral_constant<int, kChunk / 2>(), qb);
   }
   if constexpr (FUSE) {
     const double s = dot_sub_chunked<W>(
@@ -238,11 +289,8 @@ __device__ __forceinline__ void utsolve_col(const Pat &P, int j,
                                             double *y, size_t B, int inst) {
   const int q0 = uniform<W>(P.cp[j]);
   const int d = uniform<W>(P.diag[j]);
-  double s = g[(size_t)j * B + inst];
-  for (int q = q0; q < d; ++q) {
-    const int i = uniform<W>(P.ri[q]);
-    s = s - LU[(size_t)q * B + inst] * y[(size_t)i * B + inst];
-  }
+  const double s =
+      dot_sub_csc<W>(g[(size_t)j * B + inst], P.ri, LU, y, q0, d, B, inst);
   y[(size_t)j * B + inst] = s / LU[(size_t)d * B + inst];
 }
 
@@ -254,12 +302,8 @@ __device__ __forceinline__ void ltsolve_col(const Pat &P, int j,
                                             size_t B, int inst) {
   const int d = uniform<W>(P.diag[j]);
   const int q1 = uniform<W>(P.cp[j + 1]);
-  double s = y[(size_t)j * B + inst];
-  for (int q = d + 1; q < q1; ++q) {
-    const int i = uniform<W>(P.ri[q]);
-    s = s - LU[(size_t)q * B + inst] * y[(size_t)i * B + inst];
-  }
-  y[(size_t)j * B + inst] = s;
+  y[(size_t)j * B + inst] =
+      dot_sub_csc<W>(y[(size_t)j * B + inst], P.ri, LU, y, d + 1, q1, B, inst);
 }
 
 
@@ -295,6 +339,7 @@ struct ColVals {
   double a0, a1, a2, a3, bb;
 };
 
+template <bool FUSE>
 __device__ __forceinline__ void load_col(ColVals &c, int slot0, int col, int ne,
                                          const double *A, const double *b,
                                          size_t B, int inst) {
@@ -302,7 +347,7 @@ __device__ __forceinline__ void load_col(ColVals &c, int slot0, int col, int ne,
   c.a1 = ne > 1 ? A[(size_t)(slot0 + 1) * B + inst] : 0.0;
   c.a2 = ne > 2 ? A[(size_t)(slot0 + 2) * B + inst] : 0.0;
   c.a3 = ne > 3 ? A[(size_t)(slot0 + 3) * B + inst] : 0.0;
-  c.bb = b[(size_t)col * B + inst];
+  c.bb = FUSE ? b[(size_t)col * B + inst] : 0.0;
 }
 
 // A segment whose operands are all forwarded from the previous column runs
@@ -311,7 +356,7 @@ __device__ __forceinline__ void load_col(ColVals &c, int slot0, int col, int ne,
 // are unconditional in NE, the loop is unrolled three times so no register
 // copies of in-flight loads are needed): the compiler's counter-based waits
 // then wait for the column being computed only, not for the prefetches.
-template <int NE>
+template <int NE, bool FUSE>
 __device__ __forceinline__ void pure_load_col(ColVals &c, int slot0, int col,
                                               const double *A,
                                               const double *b, size_t B,
@@ -323,10 +368,11 @@ __device__ __forceinline__ void pure_load_col(ColVals &c, int slot0, int col,
     c.a2 = A[(size_t)(slot0 + 2) * B + inst];
   if constexpr (NE > 3)
     c.a3 = A[(size_t)(slot0 + 3) * B + inst];
-  c.bb = b[(size_t)col * B + inst];
+  if constexpr (FUSE)
+    c.bb = b[(size_t)col * B + inst];
 }
 
-template <int NE>
+template <int NE, bool FUSE>
 __device__ __forceinline__ void pure_lu_col(const ColVals &c, int slot0,
                                             int col, const int (&f)[16],
                                             double *LU, double *x, size_t B,
@@ -367,18 +413,20 @@ __device__ __forceinline__ void pure_lu_col(const ColVals &c, int slot0,
     LU[(size_t)(slot0 + 2) * B + inst] = v2;
   if constexpr (NE > 3)
     LU[(size_t)(slot0 + 3) * B + inst] = v3;
-  double y = c.bb;
-  if (nlr > 0)
-    y = y - sel4(p0, p1, p2, p3, -1 - f[12]) * yprev;
-  x[(size_t)col * B + inst] = y;
-  yprev = y;
+  if constexpr (FUSE) {
+    double y = c.bb;
+    if (nlr > 0)
+      y = y - sel4(p0, p1, p2, p3, -1 - f[12]) * yprev;
+    x[(size_t)col * B + inst] = y;
+    yprev = y;
+  }
   p0 = v0;
   p1 = v1;
   p2 = v2;
   p3 = v3;
 }
 
-template <int NE>
+template <int NE, bool FUSE>
 __device__ __forceinline__ void pure_lu_segment(
     const int (&f)[16], int len, int dslot, int dcol, const double *A,
     double *LU, const double *b, double *x, size_t B, int inst, double &p0,
@@ -386,11 +434,12 @@ __device__ __forceinline__ void pure_lu_segment(
   ColVals s0, s1, s2;
   auto ld = [&](ColVals &c, int i) {
     i = min(i, len - 1);
-    pure_load_col<NE>(c, f[0] + i * dslot, f[1] + i * dcol, A, b, B, inst);
+    pure_load_col<NE, FUSE>(c, f[0] + i * dslot, f[1] + i * dcol, A, b, B,
+                            inst);
   };
   auto run = [&](const ColVals &c, int i) {
-    pure_lu_col<NE>(c, f[0] + i * dslot, f[1] + i * dcol, f, LU, x, B, inst,
-                    p0, p1, p2, p3, yprev, badcol);
+    pure_lu_col<NE, FUSE>(c, f[0] + i * dslot, f[1] + i * dcol, f, LU, x, B,
+                          inst, p0, p1, p2, p3, yprev, badcol);
   };
   ld(s0, 0);
   ld(s1, 1);
@@ -415,18 +464,23 @@ struct RowVals {
   double y, dv, u0, u1;
 };
 
-template <int NT>
-__device__ __forceinline__ void pure_u_segment(const int (&f)[8], int len,
-                                               int drow, int ddiag,
-                                               const double *LU, double *x,
-                                               size_t B, int inst,
-                                               double &xprev) {
+// Row sweeps share one register program: x_i = (src_i - sum_j T(i,j) x_j)
+// [/ T(i,i) when DIV], T = U (backward, src = x in place, DIV) or unit L
+// (forward, src = b).  Records hold slots relative to the row's diagonal slot.
+template <int NT, bool DIV>
+__device__ __forceinline__ void pure_row_segment(const int (&f)[8], int len,
+                                                 int drow, int ddiag,
+                                                 const double *LU,
+                                                 const double *src, double *x,
+                                                 size_t B, int inst,
+                                                 double &xprev) {
   RowVals s0, s1, s2;
   auto ld = [&](RowVals &r, int i) {
     i = min(i, len - 1);
     const int row = f[0] + i * drow, dslot = f[1] + i * ddiag;
-    r.y = x[(size_t)row * B + inst];
-    r.dv = LU[(size_t)dslot * B + inst];
+    r.y = src[(size_t)row * B + inst];
+    if constexpr (DIV)
+      r.dv = LU[(size_t)dslot * B + inst];
     if constexpr (NT > 0)
       r.u0 = LU[(size_t)(dslot + f[4]) * B + inst];
   };
@@ -434,7 +488,7 @@ __device__ __forceinline__ void pure_u_segment(const int (&f)[8], int len,
     double s = r.y;
     if constexpr (NT > 0)
       s = s - r.u0 * xprev;
-    const double xi = s / r.dv;
+    const double xi = DIV ? s / r.dv : s;
     x[(size_t)(f[0] + i * drow) * B + inst] = xi;
     xprev = xi;
   };
@@ -456,7 +510,7 @@ __device__ __forceinline__ void pure_u_segment(const int (&f)[8], int len,
   }
 }
 
-template <int W>
+template <int W, bool FUSE>
 __device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
                                               const double *A, double *LU,
                                               const double *b, double *x,
@@ -472,23 +526,23 @@ __device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
     const int meta = f[2];
     const int ne = meta & 7, d = (meta >> 3) & 7, nops = (meta >> 6) & 7,
               nlr = (meta >> 9) & 3;
-    if (uniform<W>(sg[3])) {
+    if (uniform<W>(sg[3]) & (FUSE ? 1 : 2)) {
       int badcol = -1;
       switch (ne) {
       case 1:
-        pure_lu_segment<1>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
+        pure_lu_segment<1, FUSE>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
                            p2, p3, yprev, badcol);
         break;
       case 2:
-        pure_lu_segment<2>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
+        pure_lu_segment<2, FUSE>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
                            p2, p3, yprev, badcol);
         break;
       case 3:
-        pure_lu_segment<3>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
+        pure_lu_segment<3, FUSE>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
                            p2, p3, yprev, badcol);
         break;
       default:
-        pure_lu_segment<4>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
+        pure_lu_segment<4, FUSE>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
                            p2, p3, yprev, badcol);
         break;
       }
@@ -498,13 +552,13 @@ __device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
       continue;
     }
     ColVals c0, c1, c2;
-    load_col(c0, f[0], f[1], ne, A, b, B, inst);
+    load_col<FUSE>(c0, f[0], f[1], ne, A, b, B, inst);
     if (len > 1)
-      load_col(c1, f[0] + dslot, f[1] + dcol, ne, A, b, B, inst);
+      load_col<FUSE>(c1, f[0] + dslot, f[1] + dcol, ne, A, b, B, inst);
     for (int i = 0; i < len; ++i) {
       if (i + 2 < len)
-        load_col(c2, f[0] + (i + 2) * dslot, f[1] + (i + 2) * dcol, ne, A, b,
-                 B, inst);
+        load_col<FUSE>(c2, f[0] + (i + 2) * dslot, f[1] + (i + 2) * dcol, ne,
+                       A, b, B, inst);
       const int slot0 = f[0] + i * dslot, col = f[1] + i * dcol;
       double v0 = c0.a0, v1 = c0.a1, v2 = c0.a2, v3 = c0.a3;
 #pragma unroll
@@ -538,20 +592,21 @@ __device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
         LU[(size_t)(slot0 + 2) * B + inst] = v2;
       if (ne > 3)
         LU[(size_t)(slot0 + 3) * B + inst] = v3;
-      // forward substitution of row col
-      double y = c0.bb;
+      if constexpr (FUSE) { // forward substitution of row col
+        double y = c0.bb;
 #pragma unroll
-      for (int k = 0; k < 2; ++k) {
-        if (k < nlr) {
-          const int lsrc = f[12 + 2 * k], ysrc = f[13 + 2 * k];
-          const double l = lsrc >= 0 ? LU[(size_t)lsrc * B + inst]
-                                     : sel4(p0, p1, p2, p3, -1 - lsrc);
-          const double yj = ysrc >= 0 ? x[(size_t)ysrc * B + inst] : yprev;
-          y = y - l * yj;
+        for (int k = 0; k < 2; ++k) {
+          if (k < nlr) {
+            const int lsrc = f[12 + 2 * k], ysrc = f[13 + 2 * k];
+            const double l = lsrc >= 0 ? LU[(size_t)lsrc * B + inst]
+                                       : sel4(p0, p1, p2, p3, -1 - lsrc);
+            const double yj = ysrc >= 0 ? x[(size_t)ysrc * B + inst] : yprev;
+            y = y - l * yj;
+          }
         }
+        x[(size_t)col * B + inst] = y;
+        yprev = y;
       }
-      x[(size_t)col * B + inst] = y;
-      yprev = y;
       p0 = v0;
       p1 = v1;
       p2 = v2;
@@ -563,51 +618,58 @@ __device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
   }
 }
 
+template <bool DIV>
 __device__ __forceinline__ void load_row(RowVals &v, int row, int dslot, int nt,
                                          int rel0, int rel1, const double *LU,
-                                         const double *x, size_t B, int inst) {
-  v.y = x[(size_t)row * B + inst];
-  v.dv = LU[(size_t)dslot * B + inst];
+                                         const double *src, size_t B,
+                                         int inst) {
+  v.y = src[(size_t)row * B + inst];
+  v.dv = DIV ? LU[(size_t)dslot * B + inst] : 1.0;
   v.u0 = nt > 0 ? LU[(size_t)(dslot + rel0) * B + inst] : 0.0;
   v.u1 = nt > 1 ? LU[(size_t)(dslot + rel1) * B + inst] : 0.0;
 }
 
-template <int W>
-__device__ __forceinline__ void fast_u_chain(const Pat &P, int t0, int t1,
-                                             const double *LU, double *x,
-                                             size_t B, int inst) {
+template <int W, bool DIV>
+__device__ __forceinline__ void fast_row_chain(const int *rec, const int *seg,
+                                               int t0, int t1,
+                                               const double *LU,
+                                               const double *src, double *x,
+                                               size_t B, int inst) {
   double xprev = 0;
   for (int t = t0; t < t1;) {
-    const int *sg = P.u_seg + (size_t)t * 4;
+    const int *sg = seg + (size_t)t * 4;
     const int len = uniform<W>(sg[0]);
     const int drow = uniform<W>(sg[1]);
     const int ddiag = uniform<W>(sg[2]);
     int f[8];
-    load_record<W>(P.u_rec + (size_t)t * 8, f);
+    load_record<W>(rec + (size_t)t * 8, f);
     const int nt = f[2];
     if (uniform<W>(sg[3])) {
       if (nt == 0)
-        pure_u_segment<0>(f, len, drow, ddiag, LU, x, B, inst, xprev);
+        pure_row_segment<0, DIV>(f, len, drow, ddiag, LU, src, x, B, inst,
+                                 xprev);
       else
-        pure_u_segment<1>(f, len, drow, ddiag, LU, x, B, inst, xprev);
+        pure_row_segment<1, DIV>(f, len, drow, ddiag, LU, src, x, B, inst,
+                                 xprev);
       t += len;
       continue;
     }
     RowVals r0, r1, r2;
-    load_row(r0, f[0], f[1], nt, f[4], f[6], LU, x, B, inst);
+    load_row<DIV>(r0, f[0], f[1], nt, f[4], f[6], LU, src, B, inst);
     if (len > 1)
-      load_row(r1, f[0] + drow, f[1] + ddiag, nt, f[4], f[6], LU, x, B, inst);
+      load_row<DIV>(r1, f[0] + drow, f[1] + ddiag, nt, f[4], f[6], LU, src, B,
+                    inst);
     for (int i = 0; i < len; ++i) {
       if (i + 2 < len)
-        load_row(r2, f[0] + (i + 2) * drow, f[1] + (i + 2) * ddiag, nt, f[4],
-                 f[6], LU, x, B, inst);
+        load_row<DIV>(r2, f[0] + (i + 2) * drow, f[1] + (i + 2) * ddiag, nt,
+                      f[4], f[6], LU, src, B, inst);
       const int row = f[0] + i * drow;
       double s = r0.y;
       if (nt > 0)
         s = s - r0.u0 * (f[5] >= 0 ? x[(size_t)f[5] * B + inst] : xprev);
       if (nt > 1)
         s = s - r0.u1 * (f[7] >= 0 ? x[(size_t)f[7] * B + inst] : xprev);
-      const double xi = s / r0.dv;
+      const double xi = DIV ? s / r0.dv : s;
       x[(size_t)row * B + inst] = xi;
       xprev = xi;
       r0 = r1;
@@ -626,37 +688,46 @@ __device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
                                           int inst) {
   const int t0 = uniform<W>(L.chain_col_ptr[chain]);
   const int t1 = uniform<W>(L.chain_col_ptr[chain + 1]);
-  if constexpr (OP == OP_REFACTOR_FUSED) {
+  if constexpr (OP == OP_REFACTOR_FUSED || OP == OP_REFACTOR) {
+    constexpr bool FUSE = OP == OP_REFACTOR_FUSED;
     if (uniform<W>(P.lu_fast[chain])) {
-      fast_lu_chain<W>(P, t0, t1, A, LU, b, x, B, inst);
+      fast_lu_chain<W, FUSE>(P, t0, t1, A, LU, b, x, B, inst);
       return;
     }
     // mixed chain: register runs between generic columns
     for (int t = t0; t < t1;) {
       const int e = uniform<W>(P.lu_run[t]);
       if (e > t) {
-        fast_lu_chain<W>(P, t, e, A, LU, b, x, B, inst);
+        fast_lu_chain<W, FUSE>(P, t, e, A, LU, b, x, B, inst);
         t = e;
       } else {
-        refactor_column<W, true>(P, uniform<W>(L.cols[t]), A, LU, b, x, B,
+        refactor_column<W, FUSE>(P, uniform<W>(L.cols[t]), A, LU, b, x, B,
                                  inst);
         ++t;
       }
     }
     return;
   }
-  if constexpr (OP == OP_USOLVE) {
-    if (uniform<W>(P.u_fast[chain])) {
-      fast_u_chain<W>(P, t0, t1, LU, x, B, inst);
+  if constexpr (OP == OP_USOLVE || OP == OP_LSOLVE) {
+    constexpr bool DIV = OP == OP_USOLVE;
+    const int *rec = DIV ? P.u_rec : P.l_rec;
+    const int *seg = DIV ? P.u_seg : P.l_seg;
+    const double *src = DIV ? x : b;
+    if (uniform<W>((DIV ? P.u_fast : P.l_fast)[chain])) {
+      fast_row_chain<W, DIV>(rec, seg, t0, t1, LU, src, x, B, inst);
       return;
     }
+    const int *run = DIV ? P.u_run : P.l_run;
     for (int t = t0; t < t1;) {
-      const int e = uniform<W>(P.u_run[t]);
+      const int e = uniform<W>(run[t]);
       if (e > t) {
-        fast_u_chain<W>(P, t, e, LU, x, B, inst);
+        fast_row_chain<W, DIV>(rec, seg, t, e, LU, src, x, B, inst);
         t = e;
       } else {
-        usolve_row<W>(P, uniform<W>(L.cols[t]), LU, x, B, inst);
+        if constexpr (DIV)
+          usolve_row<W>(P, uniform<W>(L.cols[t]), LU, x, B, inst);
+        else
+          lsolve_row<W>(P, uniform<W>(L.cols[t]), LU, b, x, B, inst);
         ++t;
       }
     }
@@ -1008,6 +1079,10 @@ extern "C" int pvglu_plan_upload(pvglu_plan *P) {
       (st = dev_upload(d, &d->u_run, P->fast_u_run)) ||
       (st = dev_upload(d, &d->u_rec, P->fast_u_rec)) ||
       (st = dev_upload(d, &d->u_fast, P->fast_u_chain)) ||
+      (st = dev_upload(d, &d->l_rec, P->fast_l_rec)) ||
+      (st = dev_upload(d, &d->l_seg, P->fast_l_seg)) ||
+      (st = dev_upload(d, &d->l_run, P->fast_l_run)) ||
+      (st = dev_upload(d, &d->l_fast, P->fast_l_chain)) ||
       (st = dev_upload(d, &d->status, std::vector<int>(4, 0)))) {
     device_plan_destroy(P);
     return st;

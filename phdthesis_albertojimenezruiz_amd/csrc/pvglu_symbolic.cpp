@@ -238,24 +238,29 @@ void build_kernel_levels(const LevelSet &ls, const std::vector<int> &comp_of,
 //   [5+2k] target entry | U entry<<2 (U operands are entries of this column)
 //   [12+2k] L operand of forward term k (same encoding)
 //   [13+2k] y operand: >=0 row, -1 the previous column's y (a register)
-// Backward record (8 ints) for the row at kernel position t of the U levels:
+// Row record (8 ints) for the row at kernel position t of the U levels
+// (backward) or of the L levels (forward, without the division):
 //   [0] row  [1] diagonal slot  [2] nterms  [3] unused
-//   [4+2k] U slot of term k minus the diagonal slot
+//   [4+2k] U (L) slot of term k minus the diagonal slot
 //   [5+2k] x operand: >=0 row, -1 previous row's x
 // Strided segments: consecutive positions of a run whose records agree in
 // every field but [0] and [1], which advance by constant steps (a series
 // string: the same tridiagonal elimination step column after column), share
 // one record; seg[4t..4t+2] = (length, step of [0], step of [1]) at each
 // segment start, so the kernel loads one record per segment and computes the
-// slots and rows of the following columns; seg[4t+3] flags a segment whose
+// slots and rows of the following columns; seg[4t+3] flags segments whose
 // operands all come from the previous column's registers.
 // run[t] = end position of the register run that contains t (0 if t does not
 // qualify); chain flag = 1 when one run covers the whole chain.
 static constexpr int kFastMaxE = 4, kFastMaxOps = 4, kFastMaxLr = 2;
 
+// With the fused forward substitution the record also carries the L row of
+// the column (at most kFastMaxLr terms); without it the row is left out.
 static bool lu_column_qualifies(const pvglu_plan *P, int c) {
   const int q0 = P->cp[c], ne = P->cp[c + 1] - q0;
-  if (ne > kFastMaxE || P->lrow_ptr[c + 1] - P->lrow_ptr[c] > kFastMaxLr)
+  if (ne > kFastMaxE)
+    return false;
+  if (P->fuse_lsolve && P->lrow_ptr[c + 1] - P->lrow_ptr[c] > kFastMaxLr)
     return false;
   return P->op_ptr[q0 + ne] - P->op_ptr[q0] <= kFastMaxOps;
 }
@@ -274,7 +279,7 @@ static void encode_lu_record(const pvglu_plan *P, int c, int prev, int *r) {
       r[4 + 2 * nops] = lsrc_of(P->op_l[o]);
       r[5 + 2 * nops] = (q - q0) | ((P->op_u[o] - q0) << 2);
     }
-  const int nlr = P->lrow_ptr[c + 1] - P->lrow_ptr[c];
+  const int nlr = P->fuse_lsolve ? P->lrow_ptr[c + 1] - P->lrow_ptr[c] : 0;
   for (int k = 0; k < nlr; k++) {
     const int j = P->lrow_col[P->lrow_ptr[c] + k];
     r[12 + 2 * k] = lsrc_of(P->lrow_slot[P->lrow_ptr[c] + k]);
@@ -283,6 +288,18 @@ static void encode_lu_record(const pvglu_plan *P, int c, int prev, int *r) {
   r[0] = q0;
   r[1] = c;
   r[2] = ne | (d << 3) | (nops << 6) | (nlr << 9);
+}
+
+static void encode_l_record(const pvglu_plan *P, int i, int prev, int *r) {
+  const int nt = P->lrow_ptr[i + 1] - P->lrow_ptr[i];
+  r[0] = i;
+  r[1] = P->diag_slot[i];
+  r[2] = nt;
+  for (int k = 0; k < nt; k++) {
+    const int j = P->lrow_col[P->lrow_ptr[i] + k];
+    r[4 + 2 * k] = P->lrow_slot[P->lrow_ptr[i] + k] - P->diag_slot[i];
+    r[5 + 2 * k] = (j == prev) ? -1 : j;
+  }
 }
 
 static void encode_u_record(const pvglu_plan *P, int i, int prev, int *r) {
@@ -369,7 +386,7 @@ static void compile_segments(const std::vector<int> &rec, int rec_ints,
       seg[(size_t)u * 4] = len;
       seg[(size_t)u * 4 + 1] = d0;
       seg[(size_t)u * 4 + 2] = d1;
-      seg[(size_t)u * 4 + 3] = forwarded(&rec[(size_t)u * rec_ints]) ? 1 : 0;
+      seg[(size_t)u * 4 + 3] = forwarded(&rec[(size_t)u * rec_ints]);
       u += len;
     }
     t = e;
@@ -377,19 +394,10 @@ static void compile_segments(const std::vector<int> &rec, int rec_ints,
 }
 
 void compile_fast_programs(pvglu_plan *P) {
-  if (P->fuse_lsolve) {
-    P->fast_lu_chains = compile_runs(
-        P->k_lu, P->n, 16,
-        [&](int c) { return lu_column_qualifies(P, c); },
-        [&](int c, int prev, int *r) { encode_lu_record(P, c, prev, r); },
-        P->fast_lu_rec, P->fast_lu_run, P->fast_lu_chain);
-  } else {
-    // the register program computes the fused forward substitution
-    P->fast_lu_rec.assign((size_t)std::max(P->n, 1) * 16, 0);
-    P->fast_lu_run.assign(std::max(P->n, 1), 0);
-    P->fast_lu_chain.assign(std::max((int)P->k_lu.chain_col_ptr.size() - 1, 1), 0);
-    P->fast_lu_chains = 0;
-  }
+  P->fast_lu_chains = compile_runs(
+      P->k_lu, P->n, 16, [&](int c) { return lu_column_qualifies(P, c); },
+      [&](int c, int prev, int *r) { encode_lu_record(P, c, prev, r); },
+      P->fast_lu_rec, P->fast_lu_run, P->fast_lu_chain);
   P->fast_u_chains = compile_runs(
       P->k_u, P->n, 8,
       [&](int i) {
@@ -397,23 +405,35 @@ void compile_fast_programs(pvglu_plan *P) {
       },
       [&](int i, int prev, int *r) { encode_u_record(P, i, prev, r); },
       P->fast_u_rec, P->fast_u_run, P->fast_u_chain);
-  // seg[4t+3] = 1 when every operand of the record comes from the previous
-  // column's registers (no pattern-addressed loads inside the segment): the
-  // kernel then runs the segment as a static three-stage load pipeline.
+  // seg[4t+3] bit 1: every update operand of the record comes from the
+  // previous column's registers; bit 0: so do the forward-substitution
+  // operands.  Such a segment issues no pattern-addressed loads: the kernel
+  // runs it as a static three-stage load pipeline (bit 0 with the fused
+  // forward substitution, bit 1 for the refactor alone).
   compile_segments(P->fast_lu_rec, 16, P->fast_lu_run, P->n, P->fast_lu_seg,
                    [](const int *r) {
                      const int nops = (r[2] >> 6) & 7, nlr = (r[2] >> 9) & 3;
                      for (int k = 0; k < nops; k++)
                        if (r[4 + 2 * k] >= 0)
-                         return false;
-                     if (nlr > 1)
-                       return false;
-                     return nlr == 0 || (r[12] < 0 && r[13] == -1);
+                         return 0;
+                     const bool lrow_fwd =
+                         nlr == 0 || (nlr == 1 && r[12] < 0 && r[13] == -1);
+                     return lrow_fwd ? 3 : 2;
                    });
+  auto row_forwarded = [](const int *r) {
+    return (r[2] == 0 || (r[2] == 1 && r[5] == -1)) ? 1 : 0;
+  };
   compile_segments(P->fast_u_rec, 8, P->fast_u_run, P->n, P->fast_u_seg,
-                   [](const int *r) {
-                     return r[2] == 0 || (r[2] == 1 && r[5] == -1);
-                   });
+                   row_forwarded);
+  P->fast_l_chains = compile_runs(
+      P->k_l, P->n, 8,
+      [&](int i) {
+        return P->lrow_ptr[i + 1] - P->lrow_ptr[i] <= kFastMaxLr;
+      },
+      [&](int i, int prev, int *r) { encode_l_record(P, i, prev, r); },
+      P->fast_l_rec, P->fast_l_run, P->fast_l_chain);
+  compile_segments(P->fast_l_rec, 8, P->fast_l_run, P->n, P->fast_l_seg,
+                   row_forwarded);
   P->fast_lu_segs = P->fast_u_segs = 0;
   for (int t = 0; t < P->n; t++) {
     P->fast_lu_segs += P->fast_lu_seg[(size_t)t * 4] > 0;

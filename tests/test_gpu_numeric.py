@@ -152,6 +152,14 @@ def test_plant_refactor_solve(dev, cfg, B, sched):
     for k in check:
         assert np.array_equal(lu[:, k], want[k][0])
         assert np.array_equal(x[:, k], want[k][1])
+    # the drop-in sequence: refactor alone (register chains without the fused
+    # forward substitution), then the two substitution sweeps
+    dlu2, dx2 = torch.empty_like(dA), torch.empty_like(db)
+    plan.refactor(dA, dlu2, B)
+    plan.solve(dlu2, db, dx2, B)
+    torch.cuda.synchronize()
+    assert plan.status()[0] == 0
+    assert torch.equal(dlu2, dlu) and torch.equal(dx2, dx)
 
 
 def test_singular_pivot_is_reported(dev):
