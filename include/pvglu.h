@@ -153,9 +153,20 @@ enum {
 int pvglu_set_schedule(pvglu_plan *plan, int variant, int width);
 int pvglu_get_schedule(const pvglu_plan *plan, int *variant, int *width);
 
-/* Times the schedule candidates for this batch size on scratch buffers
- * (allocates and synchronizes: call it outside any capture) and keeps the
- * fastest.  Results are cached per (pattern hash, batch). */
+/* Component groups of the LEVEL and HYBRID schedules (1, 2 or 4; default 1).
+ * With G > 1 the independent components are split into G groups whose
+ * launches run on the plan's own side streams, forked from and joined back
+ * into the caller's stream with events (capturable), each group starting
+ * after the previous group's first level.  The side streams and events
+ * belong to the plan: two numeric calls on one plan from different host
+ * threads at the same time need G = 1.  No counterpart in the reference. */
+int pvglu_set_stream_groups(pvglu_plan *plan, int groups);
+int pvglu_get_stream_groups(const pvglu_plan *plan);
+
+/* Times the schedule candidates (variant x width x component groups) for this
+ * batch size on scratch buffers (allocates and synchronizes: call it outside
+ * any capture) and keeps the fastest.  Results are cached per (pattern hash,
+ * batch). */
 int pvglu_autotune(pvglu_plan *plan, int batch, int reps, double *best_ms);
 
 /* ------------------------------------------------------------- numeric */

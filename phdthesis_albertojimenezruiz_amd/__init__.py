@@ -72,6 +72,8 @@ def lib():
         "pvglu_plan_export": (C.c_int, [_VP, C.c_int, _P_INT]),
         "pvglu_set_schedule": (C.c_int, [_VP, C.c_int, C.c_int]),
         "pvglu_get_schedule": (C.c_int, [_VP, _P_INT, _P_INT]),
+        "pvglu_set_stream_groups": (C.c_int, [_VP, C.c_int]),
+        "pvglu_get_stream_groups": (C.c_int, [_VP]),
         "pvglu_autotune": (C.c_int, [_VP, C.c_int, C.c_int, _P_DBL]),
         "pvglu_refactor": (C.c_int, [_VP, _VP, _VP, C.c_int, _VP]),
         "pvglu_solve": (C.c_int, [_VP, _VP, _VP, _VP, C.c_int, _VP]),
@@ -168,6 +170,12 @@ class Plan:
         v, w = C.c_int(), C.c_int()
         _check(lib().pvglu_get_schedule(self.handle, C.byref(v), C.byref(w)))
         return v.value, w.value
+
+    def set_stream_groups(self, groups):
+        _check(lib().pvglu_set_stream_groups(self.handle, int(groups)))
+
+    def get_stream_groups(self):
+        return _check(lib().pvglu_get_stream_groups(self.handle))
 
     def autotune(self, batch, reps=5):
         best = C.c_double()

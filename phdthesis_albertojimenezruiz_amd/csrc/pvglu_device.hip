@@ -49,6 +49,9 @@ struct DevLevels {
   int nlev = 0, max_comp_chains = 0;
   std::vector<int> h_level_chain_ptr; // host copy for launch sizing
   std::vector<int> h_level_comp_chains; // widest component per level
+  // chain range [begin, end) of component group g at level lv, for 1, 2 and
+  // 4 groups: h_grp[gi][(g * nlev + lv) * 2 + {0, 1}]
+  std::vector<int> h_grp[3];
 };
 
 struct DevicePlan {
@@ -64,6 +67,12 @@ struct DevicePlan {
       *l_fast = nullptr;
   int *status = nullptr; // [code, column, instance, pad]
   std::vector<void *> allocs;
+  // component groups (first component of each group, 1 / 2 / 4 groups) and
+  // the side streams and events of the grouped schedules
+  std::vector<int> comp_bounds[3];
+  bool side_ok = false;
+  hipStream_t side[3] = {};
+  hipEvent_t ev_fork = nullptr, ev_skew[3] = {}, ev_join[3] = {};
 };
 
 } // namespace pvglu
@@ -754,8 +763,8 @@ __device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
 // group -> (instance tile, chain of the level).
 template <int W, int OP>
 __global__ void __launch_bounds__(256)
-    k_level(Pat P, Lev L, int level, int nchains, int tiles, const double *A,
-            double *LU, const double *b, double *x, int B) {
+    k_level(Pat P, Lev L, int first_chain, int nchains, int tiles,
+            const double *A, double *LU, const double *b, double *x, int B) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int group = uniform<W>(gid / W);
   const int lane = gid % W;
@@ -764,7 +773,7 @@ __global__ void __launch_bounds__(256)
   // consecutive wavefronts take consecutive instance tiles of one chain, so
   // the tiles of a slot row are streamed together
   const int tile = group % tiles;
-  const int chain = uniform<W>(L.level_chain_ptr[level]) + group / tiles;
+  const int chain = first_chain + group / tiles;
   const int inst = tile * W + lane;
   if (inst >= B)
     return;
@@ -864,9 +873,10 @@ struct LvRange {
 
 template <int W>
 __global__ void __launch_bounds__(1024)
-    k_hybrid(Pat P, Lev Llu, Lev Ll, Lev Lu, Lev Llt, LvRange R, int tiles,
-             const double *A, double *LU, const double *b, double *x, int B) {
-  const int comp = blockIdx.x / tiles, tile = blockIdx.x % tiles;
+    k_hybrid(Pat P, Lev Llu, Lev Ll, Lev Lu, Lev Llt, LvRange R, int comp0,
+             int tiles, const double *A, double *LU, const double *b,
+             double *x, int B) {
+  const int comp = comp0 + blockIdx.x / tiles, tile = blockIdx.x % tiles;
   const int slots = blockDim.x / W;
   const int slot = uniform<W>((int)threadIdx.x / W);
   const int inst = tile * W + (int)threadIdx.x % W;
@@ -1015,6 +1025,17 @@ void device_plan_destroy(pvglu_plan *plan) {
   }
   for (void *p : plan->dev->allocs)
     (void)hipFree(p);
+  DevicePlan *d = plan->dev;
+  for (int g = 0; g < 3; ++g) {
+    if (d->side[g])
+      (void)hipStreamDestroy(d->side[g]);
+    if (d->ev_skew[g])
+      (void)hipEventDestroy(d->ev_skew[g]);
+    if (d->ev_join[g])
+      (void)hipEventDestroy(d->ev_join[g]);
+  }
+  if (d->ev_fork)
+    (void)hipEventDestroy(d->ev_fork);
   if (switched)
     (void)hipSetDevice(cur);
   delete plan->dev;
@@ -1042,6 +1063,70 @@ extern "C" int pvglu_device_count(void) {
 }
 
 extern "C" const char *pvglu_version(void) { return "pvglu 1.0 gfx950"; }
+
+// Component groups: G runs of consecutive components with about n / G
+// columns each; bounds[g] = first component of group g.
+static std::vector<int> component_bounds(const pvglu_plan *P, int G) {
+  std::vector<int> cols(P->ncomp, 0);
+  for (int c : P->comp_of)
+    cols[c]++;
+  std::vector<int> b(G + 1, P->ncomp);
+  b[0] = 0;
+  long acc = 0;
+  int g = 1;
+  for (int c = 0; c < P->ncomp && g < G; ++c) {
+    acc += cols[c];
+    while (g < G && acc * G >= (long)P->n * g)
+      b[g++] = c + 1;
+  }
+  for (int k = 1; k <= G; ++k)
+    b[k] = std::max(b[k], b[k - 1]);
+  return b;
+}
+
+// Chain ranges of each group at each level.  Chains of a level are ordered
+// by component, so a group of consecutive components owns a contiguous range.
+static void group_ranges(const KernelLevels &k, const std::vector<int> &bounds,
+                         int nlev, std::vector<int> &out) {
+  const int G = (int)bounds.size() - 1;
+  out.assign((size_t)G * std::max(nlev, 0) * 2, 0);
+  for (int g = 0; g < G; ++g)
+    for (int lv = 0; lv < nlev; ++lv) {
+      int lo = 1 << 30, hi = -1;
+      for (int c = bounds[g]; c < bounds[g + 1]; ++c)
+        for (int e = k.comp_ptr[c]; e < k.comp_ptr[c + 1]; ++e)
+          if (k.comp_lv_level[e] == lv) {
+            lo = std::min(lo, k.comp_lv_begin[e]);
+            hi = std::max(hi, k.comp_lv_end[e]);
+          }
+      if (hi > lo) {
+        out[((size_t)g * nlev + lv) * 2] = lo;
+        out[((size_t)g * nlev + lv) * 2 + 1] = hi;
+      }
+    }
+}
+
+static int setup_groups(pvglu_plan *P) {
+  DevicePlan *d = P->dev;
+  const KernelLevels *ks[4] = {&P->k_lu, &P->k_l, &P->k_u, &P->k_lt};
+  DevLevels *ls[4] = {&d->lu, &d->l, &d->u, &d->lt};
+  for (int gi = 0; gi < 3; ++gi) {
+    d->comp_bounds[gi] = component_bounds(P, 1 << gi);
+    for (int k = 0; k < 4; ++k)
+      group_ranges(*ks[k], d->comp_bounds[gi], ls[k]->nlev, ls[k]->h_grp[gi]);
+  }
+  if (P->ncomp < 2)
+    return PVGLU_OK;
+  for (int g = 0; g < 3; ++g)
+    HIP_TRY(hipStreamCreateWithFlags(&d->side[g], hipStreamNonBlocking));
+  HIP_TRY(hipEventCreateWithFlags(&d->ev_fork, hipEventDisableTiming));
+  for (int g = 0; g < 3; ++g) {
+    HIP_TRY(hipEventCreateWithFlags(&d->ev_skew[g], hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&d->ev_join[g], hipEventDisableTiming));
+  }
+  d->side_ok = true;
+  return PVGLU_OK;
+}
 
 extern "C" int pvglu_plan_upload(pvglu_plan *P) {
   clear_error();
@@ -1083,7 +1168,8 @@ extern "C" int pvglu_plan_upload(pvglu_plan *P) {
       (st = dev_upload(d, &d->l_seg, P->fast_l_seg)) ||
       (st = dev_upload(d, &d->l_run, P->fast_l_run)) ||
       (st = dev_upload(d, &d->l_fast, P->fast_l_chain)) ||
-      (st = dev_upload(d, &d->status, std::vector<int>(4, 0)))) {
+      (st = dev_upload(d, &d->status, std::vector<int>(4, 0))) ||
+      (st = setup_groups(P))) {
     device_plan_destroy(P);
     return st;
   }
@@ -1117,33 +1203,14 @@ static void pick_default_schedule(pvglu_plan *P, int B, int &variant, int &W,
     threads = 1024;
 }
 
-template <int W, int OP>
-static int launch_levels_w(pvglu_plan *P, const DevLevels &L, const double *A,
-                           double *LU, const double *b, double *x, int B,
-                           hipStream_t s) {
-  const int tiles = (B + W - 1) / W;
-  const Pat pat = pat_of(P->dev);
-  const Lev lev = lev_of(L);
-  for (int lv = 0; lv < L.nlev; ++lv) {
-    const int nch = L.h_level_chain_ptr[lv + 1] - L.h_level_chain_ptr[lv];
-    const long threads = (long)nch * tiles * W;
-    const int blocks = (int)((threads + 255) / 256);
-    if (blocks == 0)
-      continue;
-    hipLaunchKernelGGL((k_level<W, OP>), dim3(blocks), dim3(256), 0, s, pat,
-                       lev, lv, nch, tiles, A, LU, b, x, B);
-  }
-  HIP_TRY(hipGetLastError());
-  return PVGLU_OK;
-}
-
-// One level with the operation chosen at run time (HYBRID schedule).
+// One level (or the chain range of one component group of it) with the
+// operation chosen at run time.
 template <int W>
-static void launch_one_level(pvglu_plan *P, const DevLevels &L, int lv, int op,
-                             const double *A, double *LU, const double *b,
-                             double *x, int B, hipStream_t s) {
+static void launch_one_level(pvglu_plan *P, const DevLevels &L, int first,
+                             int nch, int op, const double *A, double *LU,
+                             const double *b, double *x, int B,
+                             hipStream_t s) {
   const int tiles = (B + W - 1) / W;
-  const int nch = L.h_level_chain_ptr[lv + 1] - L.h_level_chain_ptr[lv];
   const long threads = (long)nch * tiles * W;
   const int blocks = (int)((threads + 255) / 256);
   if (blocks == 0)
@@ -1152,7 +1219,7 @@ static void launch_one_level(pvglu_plan *P, const DevLevels &L, int lv, int op,
   const Lev lev = lev_of(L);
 #define PVGLU_LVL(OPC)                                                         \
   hipLaunchKernelGGL((k_level<W, OPC>), dim3(blocks), dim3(256), 0, s, pat,    \
-                     lev, lv, nch, tiles, A, LU, b, x, B)
+                     lev, first, nch, tiles, A, LU, b, x, B)
   switch (op) {
   case OP_REFACTOR: PVGLU_LVL(OP_REFACTOR); break;
   case OP_REFACTOR_FUSED: PVGLU_LVL(OP_REFACTOR_FUSED); break;
@@ -1167,11 +1234,25 @@ static void launch_one_level(pvglu_plan *P, const DevLevels &L, int lv, int op,
 // Chains per component at or below which a level counts as narrow.
 constexpr int kNarrowChains = 4;
 
+static int group_index(int G) { return G >= 4 ? 2 : (G == 2 ? 1 : 0); }
+
+// LEVEL and HYBRID launches.  The independent components are split into G
+// groups of consecutive components (G = 1, 2, 4); group g's launches go to
+// stream g (the caller's stream for g = 0, the plan's side streams
+// otherwise), forked from and joined back into the caller's stream by
+// events.  Group g+1 starts after group g's first level, so one group's
+// latency-bound narrow levels run while the next group's wide levels use
+// the memory bandwidth.  HYBRID merges each run of two or more narrow steps
+// of a group into one component-style launch.
 template <int W>
-static int launch_hybrid_w(pvglu_plan *P, int mode, const double *A,
-                           double *LU, const double *b, double *x, int B,
-                           hipStream_t s) {
+static int launch_steps_w(pvglu_plan *P, int mode, bool hybrid, int G,
+                          const double *A, double *LU, const double *b,
+                          double *x, int B, hipStream_t s) {
   DevicePlan *d = P->dev;
+  if (G > 1 && !d->side_ok)
+    G = 1;
+  const int gi = group_index(G);
+  G = 1 << gi;
   const DevLevels *sets[4] = {&d->lu, &d->l, &d->u, &d->lt};
   int ops[4] = {-1, -1, -1, -1};
   const bool fused = mode == 1 && P->fuse_lsolve;
@@ -1194,73 +1275,85 @@ static int launch_hybrid_w(pvglu_plan *P, int mode, const double *A,
     if (ops[k] >= 0)
       for (int lv = 0; lv < sets[k]->nlev; ++lv)
         steps.push_back({k, lv, sets[k]->h_level_comp_chains[lv]});
+  const int nsteps = (int)steps.size();
   const int tiles = (B + W - 1) / W;
   const Pat pat = pat_of(d);
-  for (size_t i = 0; i < steps.size();) {
-    size_t j = i;
-    int widest = 1;
-    while (j < steps.size() && steps[j].width <= kNarrowChains) {
-      widest = std::max(widest, steps[j].width);
-      ++j;
+  if (G > 1) {
+    HIP_TRY(hipEventRecord(d->ev_fork, s));
+    for (int g = 1; g < G; ++g)
+      HIP_TRY(hipStreamWaitEvent(d->side[g - 1], d->ev_fork, 0));
+  }
+  for (int g = 0; g < G; ++g) {
+    hipStream_t st = g == 0 ? s : d->side[g - 1];
+    if (g > 0)
+      HIP_TRY(hipStreamWaitEvent(st, d->ev_skew[g - 1], 0));
+    const int c0 = d->comp_bounds[gi][g], c1 = d->comp_bounds[gi][g + 1];
+    bool skewed = false;
+    for (int i = 0; i < nsteps;) {
+      int j = i, widest = 1;
+      if (hybrid)
+        while (j < nsteps && steps[j].width <= kNarrowChains) {
+          widest = std::max(widest, steps[j].width);
+          ++j;
+        }
+      if (j - i < 2) {
+        const Step &sp = steps[i];
+        const DevLevels &L = *sets[sp.set];
+        const int *r = &L.h_grp[gi][((size_t)g * L.nlev + sp.lv) * 2];
+        launch_one_level<W>(P, L, r[0], r[1] - r[0], ops[sp.set], A, LU, b, x,
+                            B, st);
+        ++i;
+      } else {
+        LvRange R;
+        for (int k = 0; k < 4; ++k) {
+          R.lo[k] = 0;
+          R.hi[k] = 0;
+          R.op[k] = ops[k] < 0 ? 0 : ops[k];
+        }
+        for (int t = i; t < j; ++t) {
+          const Step &sp = steps[t];
+          if (R.lo[sp.set] == R.hi[sp.set])
+            R.lo[sp.set] = sp.lv;
+          R.hi[sp.set] = sp.lv + 1;
+        }
+        int slots = 1;
+        while (slots < widest)
+          slots *= 2;
+        const int threads = std::max(64, slots * W);
+        if (c1 > c0)
+          hipLaunchKernelGGL((k_hybrid<W>), dim3((c1 - c0) * tiles),
+                             dim3(threads), 0, st, pat, lev_of(d->lu),
+                             lev_of(d->l), lev_of(d->u), lev_of(d->lt), R, c0,
+                             tiles, A, LU, b, x, B);
+        i = j;
+      }
+      if (g + 1 < G && !skewed) {
+        // the next group starts once this group's first step is done
+        HIP_TRY(hipEventRecord(d->ev_skew[g], st));
+        skewed = true;
+      }
     }
-    if (j - i < 2) {
-      const Step &st = steps[i];
-      launch_one_level<W>(P, *sets[st.set], st.lv, ops[st.set], A, LU, b, x, B,
-                          s);
-      ++i;
-      continue;
-    }
-    LvRange R;
-    for (int k = 0; k < 4; ++k) {
-      R.lo[k] = 0;
-      R.hi[k] = 0;
-      R.op[k] = ops[k] < 0 ? 0 : ops[k];
-    }
-    for (size_t t = i; t < j; ++t) {
-      const Step &st = steps[t];
-      if (R.lo[st.set] == R.hi[st.set])
-        R.lo[st.set] = st.lv;
-      R.hi[st.set] = st.lv + 1;
-    }
-    int slots = 1;
-    while (slots < widest)
-      slots *= 2;
-    const int threads = std::max(64, slots * W);
-    hipLaunchKernelGGL((k_hybrid<W>), dim3(P->ncomp * tiles), dim3(threads), 0,
-                       s, pat, lev_of(d->lu), lev_of(d->l), lev_of(d->u),
-                       lev_of(d->lt), R, tiles, A, LU, b, x, B);
-    i = j;
   }
   HIP_TRY(hipGetLastError());
+  if (G > 1)
+    for (int g = 1; g < G; ++g) {
+      HIP_TRY(hipEventRecord(d->ev_join[g - 1], d->side[g - 1]));
+      HIP_TRY(hipStreamWaitEvent(s, d->ev_join[g - 1], 0));
+    }
   return PVGLU_OK;
 }
 
-static int launch_hybrid(pvglu_plan *P, int W, int mode, const double *A,
-                         double *LU, const double *b, double *x, int B,
-                         hipStream_t s) {
+static int launch_steps(pvglu_plan *P, int W, int mode, bool hybrid, int G,
+                        const double *A, double *LU, const double *b, double *x,
+                        int B, hipStream_t s) {
   switch (W) {
-  case 1: return launch_hybrid_w<1>(P, mode, A, LU, b, x, B, s);
-  case 2: return launch_hybrid_w<2>(P, mode, A, LU, b, x, B, s);
-  case 4: return launch_hybrid_w<4>(P, mode, A, LU, b, x, B, s);
-  case 8: return launch_hybrid_w<8>(P, mode, A, LU, b, x, B, s);
-  case 16: return launch_hybrid_w<16>(P, mode, A, LU, b, x, B, s);
-  case 32: return launch_hybrid_w<32>(P, mode, A, LU, b, x, B, s);
-  default: return launch_hybrid_w<64>(P, mode, A, LU, b, x, B, s);
-  }
-}
-
-template <int OP>
-static int launch_levels(pvglu_plan *P, const DevLevels &L, int W,
-                         const double *A, double *LU, const double *b,
-                         double *x, int B, hipStream_t s) {
-  switch (W) {
-  case 1: return launch_levels_w<1, OP>(P, L, A, LU, b, x, B, s);
-  case 2: return launch_levels_w<2, OP>(P, L, A, LU, b, x, B, s);
-  case 4: return launch_levels_w<4, OP>(P, L, A, LU, b, x, B, s);
-  case 8: return launch_levels_w<8, OP>(P, L, A, LU, b, x, B, s);
-  case 16: return launch_levels_w<16, OP>(P, L, A, LU, b, x, B, s);
-  case 32: return launch_levels_w<32, OP>(P, L, A, LU, b, x, B, s);
-  default: return launch_levels_w<64, OP>(P, L, A, LU, b, x, B, s);
+  case 1: return launch_steps_w<1>(P, mode, hybrid, G, A, LU, b, x, B, s);
+  case 2: return launch_steps_w<2>(P, mode, hybrid, G, A, LU, b, x, B, s);
+  case 4: return launch_steps_w<4>(P, mode, hybrid, G, A, LU, b, x, B, s);
+  case 8: return launch_steps_w<8>(P, mode, hybrid, G, A, LU, b, x, B, s);
+  case 16: return launch_steps_w<16>(P, mode, hybrid, G, A, LU, b, x, B, s);
+  case 32: return launch_steps_w<32>(P, mode, hybrid, G, A, LU, b, x, B, s);
+  default: return launch_steps_w<64>(P, mode, hybrid, G, A, LU, b, x, B, s);
   }
 }
 
@@ -1309,32 +1402,8 @@ static int run_numeric(pvglu_plan *P, int mode, const double *A, double *LU,
   pick_default_schedule(P, B, variant, W, threads);
   if (variant == PVGLU_SCHED_COMPONENT)
     return launch_component(P, W, mode, threads, A, LU, b, x, B, s);
-  if (variant == PVGLU_SCHED_HYBRID)
-    return launch_hybrid(P, W, mode, (double *)A, LU, b, x, B, s);
-  DevicePlan *d = P->dev;
-  if (mode == 3) {
-    if ((st = launch_levels<OP_UT>(P, d->lu, W, nullptr, (double *)LU, b, x,
-                                   B, s)))
-      return st;
-    return launch_levels<OP_LT>(P, d->lt, W, nullptr, (double *)LU, nullptr,
-                                x, B, s);
-  }
-  if (mode <= 1) {
-    bool fused = mode == 1 && P->fuse_lsolve;
-    st = fused ? launch_levels<OP_REFACTOR_FUSED>(P, d->lu, W, A, LU, b, x, B, s)
-               : launch_levels<OP_REFACTOR>(P, d->lu, W, A, LU, b, x, B, s);
-    if (st || mode == 0)
-      return st;
-    if (!fused && (st = launch_levels<OP_LSOLVE>(P, d->l, W, nullptr, LU, b,
-                                                 x, B, s)))
-      return st;
-    return launch_levels<OP_USOLVE>(P, d->u, W, nullptr, LU, nullptr, x, B, s);
-  }
-  if ((st = launch_levels<OP_LSOLVE>(P, d->l, W, nullptr, (double *)LU, b, x,
-                                     B, s)))
-    return st;
-  return launch_levels<OP_USOLVE>(P, d->u, W, nullptr, (double *)LU, nullptr,
-                                  x, B, s);
+  return launch_steps(P, W, mode, variant == PVGLU_SCHED_HYBRID,
+                      P->sched_groups, A, LU, b, x, B, s);
 }
 
 extern "C" int pvglu_refactor(pvglu_plan *P, const double *a, double *lu,
@@ -1402,6 +1471,22 @@ extern "C" int pvglu_set_schedule(pvglu_plan *P, int variant, int width) {
   return PVGLU_OK;
 }
 
+extern "C" int pvglu_set_stream_groups(pvglu_plan *P, int groups) {
+  clear_error();
+  if (!P)
+    return fail(PVGLU_E_ARG, "set_stream_groups: null plan");
+  if (groups != 1 && groups != 2 && groups != 4)
+    return fail(PVGLU_E_ARG, "set_stream_groups: %d not 1, 2 or 4", groups);
+  P->sched_groups = groups;
+  return PVGLU_OK;
+}
+
+extern "C" int pvglu_get_stream_groups(const pvglu_plan *P) {
+  if (!P)
+    return fail(PVGLU_E_ARG, "get_stream_groups: null plan");
+  return P->sched_groups;
+}
+
 extern "C" int pvglu_get_schedule(const pvglu_plan *P, int *variant,
                                   int *width) {
   if (!P)
@@ -1462,7 +1547,10 @@ __global__ void k_tune_values(int phase, const int *cp, const int *diag, int n,
 }
 
 static std::mutex g_tune_mu;
-static std::map<std::pair<uint64_t, int>, std::pair<int, int>> g_tune_cache;
+struct TuneChoice {
+  int variant, width, groups;
+};
+static std::map<std::pair<uint64_t, int>, TuneChoice> g_tune_cache;
 
 extern "C" int pvglu_autotune(pvglu_plan *P, int batch, int reps,
                               double *best_ms) {
@@ -1478,8 +1566,9 @@ extern "C" int pvglu_autotune(pvglu_plan *P, int batch, int reps,
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tune_cache.find({P->hash, batch});
     if (it != g_tune_cache.end() && !best_ms) {
-      P->sched_variant = it->second.first;
-      P->sched_width = it->second.second;
+      P->sched_variant = it->second.variant;
+      P->sched_width = it->second.width;
+      P->sched_groups = it->second.groups;
       P->tuned_batch = batch;
       return PVGLU_OK;
     }
@@ -1509,40 +1598,48 @@ extern "C" int pvglu_autotune(pvglu_plan *P, int batch, int reps,
     hipLaunchKernelGGL(k_tune_values, dim3((unsigned)((nv + 255) / 256)),
                        dim3(256), 0, s, phase, P->dev->cp, P->dev->diag_slot,
                        P->n, a, b, batch);
-  const int save_v = P->sched_variant, save_w = P->sched_width;
+  const int save_v = P->sched_variant, save_w = P->sched_width,
+            save_g = P->sched_groups;
   double best = 1e300;
-  int bv = PVGLU_SCHED_LEVEL, bw = 1;
+  int bv = PVGLU_SCHED_LEVEL, bw = 1, bg = 1;
   const int widths[] = {1, 2, 4, 8, 16, 32, 64};
-  for (int variant = PVGLU_SCHED_LEVEL; variant <= PVGLU_SCHED_HYBRID;
+  for (int variant = PVGLU_SCHED_LEVEL; variant <= PVGLU_SCHED_HYBRID && !st;
        ++variant) {
     for (int w : widths) {
       if (w > 1 && w / 2 >= batch)
         continue; // wider tiles only idle lanes
-      P->sched_variant = variant;
-      P->sched_width = w;
-      st = pvglu_refactor_solve(P, a, lu, b, x, batch, s);
+      for (int groups = 1; groups <= 4; groups *= 2) {
+        if (groups > 1 && (variant == PVGLU_SCHED_COMPONENT ||
+                           !P->dev->side_ok || groups > P->ncomp))
+          continue;
+        P->sched_variant = variant;
+        P->sched_width = w;
+        P->sched_groups = groups;
+        st = pvglu_refactor_solve(P, a, lu, b, x, batch, s);
+        if (st)
+          break;
+        std::vector<float> t;
+        for (int r = 0; r < reps; ++r) {
+          (void)hipEventRecord(e0, s);
+          st = pvglu_refactor_solve(P, a, lu, b, x, batch, s);
+          (void)hipEventRecord(e1, s);
+          (void)hipEventSynchronize(e1);
+          float ms = 0;
+          (void)hipEventElapsedTime(&ms, e0, e1);
+          t.push_back(ms);
+        }
+        std::sort(t.begin(), t.end());
+        const double med = t[t.size() / 2];
+        if (med < best) {
+          best = med;
+          bv = variant;
+          bw = w;
+          bg = groups;
+        }
+      }
       if (st)
         break;
-      std::vector<float> t;
-      for (int r = 0; r < reps; ++r) {
-        (void)hipEventRecord(e0, s);
-        st = pvglu_refactor_solve(P, a, lu, b, x, batch, s);
-        (void)hipEventRecord(e1, s);
-        (void)hipEventSynchronize(e1);
-        float ms = 0;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        t.push_back(ms);
-      }
-      std::sort(t.begin(), t.end());
-      const double med = t[t.size() / 2];
-      if (med < best) {
-        best = med;
-        bv = variant;
-        bw = w;
-      }
     }
-    if (st)
-      break;
   }
   (void)hipStreamSynchronize(s);
   (void)hipEventDestroy(e0);
@@ -1552,6 +1649,7 @@ extern "C" int pvglu_autotune(pvglu_plan *P, int batch, int reps,
   if (st) {
     P->sched_variant = save_v;
     P->sched_width = save_w;
+    P->sched_groups = save_g;
     return st;
   }
   // clear any status the stand-in values may have recorded
@@ -1559,10 +1657,11 @@ extern "C" int pvglu_autotune(pvglu_plan *P, int batch, int reps,
   HIP_TRY(hipMemcpy(P->dev->status, zero, sizeof zero, hipMemcpyHostToDevice));
   P->sched_variant = bv;
   P->sched_width = bw;
+  P->sched_groups = bg;
   P->tuned_batch = batch;
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune_cache[{P->hash, batch}] = {bv, bw};
+    g_tune_cache[{P->hash, batch}] = {bv, bw, bg};
   }
   if (best_ms)
     *best_ms = best;

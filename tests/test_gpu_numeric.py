@@ -113,8 +113,70 @@ def test_solve_transpose_bitexact(dev, sched):
     assert np.array_equal(dy.cpu().numpy(), want)
 
 
-PLANT_SCHEDS = [(pv.SCHED_AUTO, 0), (pv.SCHED_LEVEL, 0), (pv.SCHED_COMPONENT, 0),
-                (pv.SCHED_HYBRID, 0)]
+def _multi_component_case(seed):
+    """Block-diagonal pattern (6 random blocks of unequal size, one of them a
+    single column) under a random symmetric permutation, so independent
+    components interleave in column order; filled by the oracle."""
+    rng = np.random.default_rng(seed)
+    sizes = [1] + [int(v) for v in rng.integers(5, 60, 5)]
+    n = sum(sizes)
+    rows, off = [set() for _ in range(n)], 0
+    for m in sizes:
+        blk = random_pattern(rng, m, float(rng.uniform(0.05, 0.2)), symmetric=True)
+        for i, r in enumerate(blk):
+            rows[off + i] = {off + j for j in r}
+        off += m
+    perm = rng.permutation(n)
+    prow = [set() for _ in range(n)]
+    for i, r in enumerate(rows):
+        prow[perm[i]] = {int(perm[j]) for j in r}
+    rp, ci = csr_from_sets(prow)
+    frp, fci = O.symbolic_fill(n, rp, ci)
+    return rng, n, frp, fci
+
+
+@pytest.mark.parametrize("variant", [pv.SCHED_LEVEL, pv.SCHED_HYBRID])
+@pytest.mark.parametrize("groups", [2, 4])
+@pytest.mark.parametrize("width", [1, 64])
+def test_component_groups_bitexact(dev, variant, groups, width):
+    """Component groups on side streams (fork/join by events): refactor+solve,
+    refactor then solve, and the transposed solve against the oracle."""
+    rng, n, rp, ci = _multi_component_case(11)
+    plan = pv.Plan(n, rp, ci)
+    assert plan.info("ncomp") >= 6
+    plan.set_schedule(variant, width)
+    plan.set_stream_groups(groups)
+    assert plan.get_stream_groups() == groups
+    s = O.csr_csc(n, rp, ci)
+    B = 70
+    A = np.stack([dd_values(rng, n, rp, ci)[s.csc2csr] for _ in range(B)], axis=1)
+    bb = rng.normal(size=(n, B))
+    G = rng.normal(size=(n, B))
+    dA, db, dg = _to_dev(A, dev), _to_dev(bb, dev), _to_dev(G, dev)
+    dlu, dx, dy = torch.empty_like(dA), torch.empty_like(db), torch.empty_like(dg)
+    stream = torch.cuda.Stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        plan.refactor_solve(dA, dlu, db, dx, B, stream)
+        plan.solve_transpose(dlu, dg, dy, B, stream)
+    stream.synchronize()
+    lu, x, y = dlu.cpu().numpy(), dx.cpu().numpy(), dy.cpu().numpy()
+    for k in (0, 33, B - 1):
+        wlu, wx = _oracle_lu_solve(s, A[:, k].copy(), bb[:, k].copy())
+        assert np.array_equal(lu[:, k], wlu) and np.array_equal(x[:, k], wx)
+        g = G[:, k].copy()
+        O.solve_transpose_csc(s, wlu, g)
+        assert np.array_equal(y[:, k], g)
+    dlu2, dx2 = torch.empty_like(dA), torch.empty_like(db)
+    plan.refactor(dA, dlu2, B)
+    plan.solve(dlu2, db, dx2, B)
+    torch.cuda.synchronize()
+    assert torch.equal(dlu2, dlu) and torch.equal(dx2, dx)
+
+
+# (variant, width, component groups on side streams)
+PLANT_SCHEDS = [(pv.SCHED_AUTO, 0, 1), (pv.SCHED_LEVEL, 0, 1), (pv.SCHED_COMPONENT, 0, 1),
+                (pv.SCHED_HYBRID, 0, 1), (pv.SCHED_LEVEL, 0, 2), (pv.SCHED_HYBRID, 0, 4)]
 
 
 @pytest.mark.parametrize("cfg", [(4, 10, 20), (40, 24, 28)])
@@ -131,7 +193,8 @@ def test_plant_refactor_solve(dev, cfg, B, sched):
     P = O.amd_order(n, rp0, ci0) if O.ref() is not None else np.arange(n, dtype=np.int32)
     rp, ci, _ = O.permute_sym(n, rp0, ci0, np.zeros(len(ci0)), P)
     plan = pv.Plan(n, rp, ci)
-    plan.set_schedule(*sched)
+    plan.set_schedule(*sched[:2])
+    plan.set_stream_groups(sched[2])
     s = O.csr_csc(n, rp, ci)
     Pinv = O.invert_permutation(P)
     _, _, _, V, bv = plant_values(rng, I, S, M, B)
