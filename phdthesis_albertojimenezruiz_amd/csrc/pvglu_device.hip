@@ -435,37 +435,46 @@ __device__ __forceinline__ void pure_lu_col(const ColVals &c, int slot0,
   p3 = v3;
 }
 
+// Software pipeline over the steps i of a segment: the loads of step i + D
+// are issued before step i computes, into a ring of D + 1 register buffers
+// indexed by unrolled constants (no copies of registers with loads in
+// flight); LD clamps its step to the segment.
+template <int D, class T, class LD, class RUN>
+__device__ __forceinline__ void pipeline(int len, LD ld, RUN run) {
+  T buf[D + 1];
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    ld(buf[k], k);
+  for (int i = 0;;) {
+#pragma unroll
+    for (int u = 0; u <= D; ++u) {
+      ld(buf[(u + D) % (D + 1)], i + D);
+      run(buf[u], i);
+      if (++i >= len)
+        return;
+    }
+  }
+}
+
+// Prefetch depth of the refactor and substitution register pipelines.
+constexpr int kLuDepth = 2, kRowDepth = 2;
+
 template <int NE, bool FUSE>
 __device__ __forceinline__ void pure_lu_segment(
     const int (&f)[16], int len, int dslot, int dcol, const double *A,
     double *LU, const double *b, double *x, size_t B, int inst, double &p0,
     double &p1, double &p2, double &p3, double &yprev, int &badcol) {
-  ColVals s0, s1, s2;
-  auto ld = [&](ColVals &c, int i) {
-    i = min(i, len - 1);
-    pure_load_col<NE, FUSE>(c, f[0] + i * dslot, f[1] + i * dcol, A, b, B,
-                            inst);
-  };
-  auto run = [&](const ColVals &c, int i) {
-    pure_lu_col<NE, FUSE>(c, f[0] + i * dslot, f[1] + i * dcol, f, LU, x, B,
-                          inst, p0, p1, p2, p3, yprev, badcol);
-  };
-  ld(s0, 0);
-  ld(s1, 1);
-  for (int i = 0;;) {
-    ld(s2, i + 2);
-    run(s0, i);
-    if (++i >= len)
-      break;
-    ld(s0, i + 2);
-    run(s1, i);
-    if (++i >= len)
-      break;
-    ld(s1, i + 2);
-    run(s2, i);
-    if (++i >= len)
-      break;
-  }
+  pipeline<kLuDepth, ColVals>(
+      len,
+      [&](ColVals &c, int i) {
+        i = min(i, len - 1);
+        pure_load_col<NE, FUSE>(c, f[0] + i * dslot, f[1] + i * dcol, A, b, B,
+                                inst);
+      },
+      [&](const ColVals &c, int i) {
+        pure_lu_col<NE, FUSE>(c, f[0] + i * dslot, f[1] + i * dcol, f, LU, x,
+                              B, inst, p0, p1, p2, p3, yprev, badcol);
+      });
 }
 
 // Backward-substitution values of one row: y, U(i,i) and the two U terms.
@@ -483,40 +492,25 @@ __device__ __forceinline__ void pure_row_segment(const int (&f)[8], int len,
                                                  const double *src, double *x,
                                                  size_t B, int inst,
                                                  double &xprev) {
-  RowVals s0, s1, s2;
-  auto ld = [&](RowVals &r, int i) {
-    i = min(i, len - 1);
-    const int row = f[0] + i * drow, dslot = f[1] + i * ddiag;
-    r.y = src[(size_t)row * B + inst];
-    if constexpr (DIV)
-      r.dv = LU[(size_t)dslot * B + inst];
-    if constexpr (NT > 0)
-      r.u0 = LU[(size_t)(dslot + f[4]) * B + inst];
-  };
-  auto run = [&](const RowVals &r, int i) {
-    double s = r.y;
-    if constexpr (NT > 0)
-      s = s - r.u0 * xprev;
-    const double xi = DIV ? s / r.dv : s;
-    x[(size_t)(f[0] + i * drow) * B + inst] = xi;
-    xprev = xi;
-  };
-  ld(s0, 0);
-  ld(s1, 1);
-  for (int i = 0;;) {
-    ld(s2, i + 2);
-    run(s0, i);
-    if (++i >= len)
-      break;
-    ld(s0, i + 2);
-    run(s1, i);
-    if (++i >= len)
-      break;
-    ld(s1, i + 2);
-    run(s2, i);
-    if (++i >= len)
-      break;
-  }
+  pipeline<kRowDepth, RowVals>(
+      len,
+      [&](RowVals &r, int i) {
+        i = min(i, len - 1);
+        const int row = f[0] + i * drow, dslot = f[1] + i * ddiag;
+        r.y = src[(size_t)row * B + inst];
+        if constexpr (DIV)
+          r.dv = LU[(size_t)dslot * B + inst];
+        if constexpr (NT > 0)
+          r.u0 = LU[(size_t)(dslot + f[4]) * B + inst];
+      },
+      [&](const RowVals &r, int i) {
+        double s = r.y;
+        if constexpr (NT > 0)
+          s = s - r.u0 * xprev;
+        const double xi = DIV ? s / r.dv : s;
+        x[(size_t)(f[0] + i * drow) * B + inst] = xi;
+        xprev = xi;
+      });
 }
 
 template <int W, bool FUSE>
