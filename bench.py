@@ -49,44 +49,69 @@ def parse():
     ap.add_argument("--no-plain", action="store_true", help="skip plain / drop-in legs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration)")
+    ap.add_argument("--schedule", default="level:64:1",
+                    help="variant:width:groups of the batched launch (level, component, "
+                         "hybrid), or 'auto' for the autotuner's choice")
+    ap.add_argument("--cpu-threads", type=int, default=16,
+                    help="threads of the all-cores CPU baseline (0: skip)")
     return ap.parse_args()
 
 
-def cpu_baseline(plan_arrays, A_host, b_host, budget_s=12.0):
-    """Reference CPU GLU (GLU_CSC_CPU_sequential + Substitute_LU_Vector_b_CSC_CPU
-    compiled from the reference's own sources with its host flags), 1 thread,
-    on a bounded sample of the same instances."""
+SCHED_NAMES = {"level": 1, "component": 2, "hybrid": 3}
+
+
+def _cpu_kind():
     from oracle import oracle as O
-    kind = "reference" if O.ref(fast=True) is not None else "port"
+    return "reference" if O.ref(fast=True) is not None else "port"
+
+
+def _cpu_one(s, lu, x, kind):
+    from oracle import oracle as O
+    ref = kind == "reference"
+    O.glu_csc_sequential(s, lu, use_ref=ref, fast=True)
+    O.substitute_lu_csc(s, lu, x, use_ref=ref, fast=True)
+
+
+def _cpu_run(s, AT, bT, ks, kind):
+    """refactor+solve of instances ks on one thread; each instance's values are
+    copied into a scratch vector first (the GLU works in place)."""
+    lu = np.empty(AT.shape[1])
+    x = np.empty(bT.shape[1])
+    for k in ks:
+        np.copyto(lu, AT[k])
+        np.copyto(x, bT[k])
+        _cpu_one(s, lu, x, kind)
+
+
+def cpu_baseline(plan_arrays, A_host, b_host, threads=1, budget_s=12.0):
+    """Reference CPU GLU (GLU_CSC_CPU_sequential + Substitute_LU_Vector_b_CSC_CPU
+    compiled from the reference's own sources with its host flags) on a
+    bounded sample of the same instances, on `threads` threads (the foreign
+    calls release the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+    from oracle import oracle as O
+    kind = _cpu_kind()
     s = O.CsrCsc(*plan_arrays)
     B = A_host.shape[1]
-    # calibrate on one instance, then run a sample of ~budget_s seconds
+    AT = np.ascontiguousarray(A_host.T)
+    bT = np.ascontiguousarray(b_host.T)
+    _cpu_run(s, AT, bT, [k % B for k in range(5)], kind)   # warm-up
     t0 = time.perf_counter()
-    lu = np.ascontiguousarray(A_host[:, 0])
-    x = np.ascontiguousarray(b_host[:, 0])
-    if kind == "reference":
-        O.glu_csc_sequential(s, lu, use_ref=True, fast=True)
-        O.substitute_lu_csc(s, lu, x, use_ref=True, fast=True)
+    _cpu_run(s, AT, bT, [k % B for k in range(20)], kind)
+    one = max((time.perf_counter() - t0) / 20, 1e-6)
+    count = int(min(max(threads, budget_s * threads / one), 400 * B))
+    ks = [k % B for k in range(count)]
+    t0 = time.perf_counter()
+    if threads == 1:
+        _cpu_run(s, AT, bT, ks, kind)
     else:
-        O.glu_csc_sequential(s, lu, fast=True)
-        O.substitute_lu_csc(s, lu, x, fast=True)
-    one = max(time.perf_counter() - t0, 1e-6)
-    count = int(min(max(3, budget_s / one), 20 * B))
-    cols = [k % B for k in range(count)]
-    work = [(np.ascontiguousarray(A_host[:, c]), np.ascontiguousarray(b_host[:, c]))
-            for c in cols]
-    t0 = time.perf_counter()
-    for lu, x in work:
-        if kind == "reference":
-            O.glu_csc_sequential(s, lu, use_ref=True, fast=True)
-            O.substitute_lu_csc(s, lu, x, use_ref=True, fast=True)
-        else:
-            O.glu_csc_sequential(s, lu, fast=True)
-            O.substitute_lu_csc(s, lu, x, fast=True)
+        with ThreadPoolExecutor(threads) as ex:
+            list(ex.map(lambda j: _cpu_run(s, AT, bT, ks[j::threads], kind), range(threads)))
     dt = time.perf_counter() - t0
-    return {"value": count / dt, "unit": "refactor+solve/s", "cores": 1, "kind": kind,
-            "sample": f"{count} instances of the batch (refactor+solve each), "
-                      f"{dt:.1f} s, 1 thread"}
+    return {"value": count / dt, "unit": "refactor+solve/s", "cores": threads, "kind": kind,
+            "sample": f"{count} instances of the batch (refactor+solve each, values copied "
+                      f"in per instance), {dt:.1f} s, {threads} thread"
+                      + ("s" if threads > 1 else "")}
 
 
 def main():
@@ -124,9 +149,14 @@ def main():
               f"state instead", flush=True)
         v_n = plant.v0(B)
         state = "initial Newton state (V = 600/M per module)"
-    if plan.get_schedule()[1] == 0:
+    if args.schedule == "auto":
         plan.autotune(B, 5)
-    sched = plan.get_schedule()
+    else:
+        v, w, g = args.schedule.split(":")
+        plan.set_schedule(SCHED_NAMES[v], int(w))
+        plan.set_stream_groups(int(g))
+    sched = (*plan.get_schedule(), plan.get_stream_groups())
+    sched_name = {1: "level", 2: "component", 3: "hybrid"}[sched[0]] + f":{sched[1]}:{sched[2]}"
 
     dG = torch.from_numpy(G).to(dev)
     dV = torch.from_numpy(v_n).to(dev)
@@ -185,14 +215,13 @@ def main():
     achieved = bytes_launch / (avg_launch_ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None, "kernel": "refactor+solve launch (schedule "
-                                            f"{sched[0]}/{sched[1]})",
+                "traffic": None, "kernel": f"refactor+solve launch (schedule {sched_name})",
                 "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg_launch_ms, 5)}
     tf = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf):
         try:
             t = json.load(open(tf))
-            if t.get("batch") == B and t.get("n") == n:
+            if t.get("batch") == B and t.get("n") == n and t.get("schedule") == sched_name:
                 roofline["traffic"] = t.get("bytes_per_launch")
         except (OSError, ValueError):
             pass
@@ -210,7 +239,7 @@ def main():
            "config": {"workload": f"batched: {B} cloud scenarios x {args.inverters}x"
                                   f"{args.strings}x{args.modules} plant, shared pattern",
                       "batch": B, "n": n, "nnz_lu": nnz, "ordering": "AMD",
-                      "schedule": {"variant": sched[0], "width": sched[1]},
+                      "schedule": sched_name,
                       "newton_tol_V": 1e-10, "max_abs_dx_vs_oracle": max_dx},
            "roofline": roofline}
 
@@ -259,6 +288,13 @@ def main():
         out["cpu_baseline"] = cb
         out["batched_over_cpu"] = value / cb["value"]
         print(f"[bench] cpu baseline ({cb['kind']}): {cb['value']:.4g}/s on {model}", flush=True)
+        if args.cpu_threads > 0:
+            ct = cpu_baseline(plan_arrays, A_host, b_host, threads=args.cpu_threads,
+                              budget_s=8.0)
+            ct["cpu"] = model
+            out["cpu_baseline_all_cores"] = ct
+            out["batched_over_cpu_all_cores"] = value / ct["value"]
+            print(f"[bench] cpu baseline, {ct['cores']} threads: {ct['value']:.4g}/s", flush=True)
 
     if args.copy_probe:
         src = torch.empty(1 << 27, dtype=torch.float64, device=dev)
