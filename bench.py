@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip plain / drop-in legs")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the mixed-size (10/40/100 inverters) and 400-inverter legs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration)")
     ap.add_argument("--schedule", default="level:64:1",
@@ -112,6 +114,82 @@ def cpu_baseline(plan_arrays, A_host, b_host, threads=1, budget_s=12.0):
             "sample": f"{count} instances of the batch (refactor+solve each, values copied "
                       f"in per instance), {dt:.1f} s, {threads} thread"
                       + ("s" if threads > 1 else "")}
+
+
+def _time_ms(torch, fn, reps):
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def config_legs(torch, pv, dev):
+    """Secondary workloads, reported beside the headline value:
+    mixed: 128 instances round-robin over 10 / 40 / 100-inverter plants, one
+    multi-pattern launch for refactor+solve and one for the adjoint solve;
+    large: the 400-inverter plant (269,200 unknowns), 32 scenarios, five
+    warm-started one-second time steps of batched Newton, one refactor+solve
+    launch, and the energy adjoint of one step."""
+    out = {}
+    plants = [pv.PV(I, 24, 28, seed=3) for I in (10, 40, 100)]
+    batches = [len(range(g, 128, 3)) for g in range(3)]
+    As, bs, gs, lus, xs, ys, nbytes, abytes = [], [], [], [], [], [], 0, 0
+    for g, (p, B) in enumerate(zip(plants, batches)):
+        plan = p.plan()
+        G = np.stack([p.irradiance(s, 0.0) for s in range(g, 128, 3)], axis=1)
+        A = torch.empty((plan.nnz, B), dtype=torch.float64, device=dev)
+        b = torch.empty((p.n, B), dtype=torch.float64, device=dev)
+        p.stamp(B, torch.from_numpy(G).to(dev), torch.from_numpy(p.v0(B)).to(dev), A, b, -1.0)
+        As.append(A), bs.append(b), gs.append(b.clone())
+        lus.append(torch.empty_like(A)), xs.append(torch.empty_like(b)), ys.append(torch.empty_like(b))
+        nbytes += B * (24 * plan.nnz + 16 * p.n) + 4 * plan.nnz + 8 * p.n
+        abytes += B * (8 * plan.nnz + 16 * p.n)
+    m = pv.Multi([p.plan() for p in plants], batches)
+    t_rs = _time_ms(torch, lambda: m.refactor_solve(As, lus, bs, xs), 20)
+    t_adj = _time_ms(torch, lambda: m.solve_transpose(lus, gs, ys), 20)
+    out["mixed_10_40_100"] = {
+        "instances": 128, "refactor_solve_ms": round(t_rs, 4),
+        "refactor_solve_per_s": round(128 / t_rs * 1e3, 1),
+        "refactor_solve_GBps": round(nbytes / t_rs / 1e6, 1),
+        "adjoint_ms": round(t_adj, 4), "adjoint_GBps": round(abytes / t_adj / 1e6, 1)}
+    del As, bs, gs, lus, xs, ys, m
+    big = pv.PV(400, 24, 28, seed=4)
+    B = 32
+    t0 = time.perf_counter()
+    v = None
+    iters = []
+    for t in range(6):
+        G = np.stack([big.irradiance(s, float(t)) for s in range(B)], axis=1)
+        x, v, p, st = big.newton_batched(G, v=v, tol=1e-10, ladder=False)
+        iters.append(int(st[0]))
+        if t == 0:
+            t0 = time.perf_counter()          # steps 1..5 are the warm-started ones
+    t_step = (time.perf_counter() - t0) / 5 * 1e3
+    plan = big.plan()
+    A = torch.empty((plan.nnz, B), dtype=torch.float64, device=dev)
+    b = torch.empty((big.n, B), dtype=torch.float64, device=dev)
+    big.stamp(B, torch.from_numpy(G).to(dev), torch.from_numpy(v).to(dev), A, b, -1.0)
+    lu, xx = torch.empty_like(A), torch.empty_like(b)
+    t_rs = _time_ms(torch, lambda: plan.refactor_solve(A, lu, b, xx, B), 20)
+    byts = B * (24 * plan.nnz + 16 * big.n) + 4 * plan.nnz + 8 * big.n
+    t1 = time.perf_counter()
+    big.gradients(G, v=v, tol=1e-12)
+    t_adj = (time.perf_counter() - t1) * 1e3
+    out["large_400x24x28"] = {
+        "n": big.n, "scenarios": B, "warm_step_ms": round(t_step, 2),
+        "newton_iterations_per_step": iters,
+        "refactor_solve_ms": round(t_rs, 4), "refactor_solve_GBps": round(byts / t_rs / 1e6, 1),
+        "energy_adjoint_step_ms": round(t_adj, 2),
+        "note": "warm_step_ms: batched device Newton to 1e-10 V from the previous second's "
+                "state, host loop included; energy_adjoint_step_ms: re-converge to 1e-12 V + "
+                "transposed solves + gradient kernels"}
+    return out
 
 
 def main():
@@ -295,6 +373,11 @@ def main():
             out["cpu_baseline_all_cores"] = ct
             out["batched_over_cpu_all_cores"] = value / ct["value"]
             print(f"[bench] cpu baseline, {ct['cores']} threads: {ct['value']:.4g}/s", flush=True)
+
+    if not args.no_configs:
+        out["configs"] = config_legs(torch, pv, dev)
+        for k, v in out["configs"].items():
+            print(f"[bench] {k}: {v}", flush=True)
 
     if args.copy_probe:
         src = torch.empty(1 << 27, dtype=torch.float64, device=dev)
