@@ -134,6 +134,19 @@ enum {
 int pvglu_plan_array_len(const pvglu_plan *plan, int which);
 int pvglu_plan_export(const pvglu_plan *plan, int which, int *out);
 
+/* On-disk plan: the pattern half of struct_CPU_CUDA_arrays
+ * (PV/factorize_electrical_system.h:3-61) -- every pvglu_plan_export array --
+ * and optionally the ordering (P, Prow; NULL to omit), with a checksum.
+ * Loading rebuilds the plan from the stored filled pattern and fails with
+ * PVGLU_E_ARG if any stored array differs from the rebuilt one; P / Prow (n
+ * ints each, may be NULL) receive the stored ordering (identity if none was
+ * stored).  pvglu_plan_file_n returns n from a file's header.  No counterpart
+ * in the reference, which recomputes the symbolic phase every run. */
+int pvglu_plan_save(const pvglu_plan *plan, const char *path, const int *P,
+                    const int *Prow);
+int pvglu_plan_file_n(const char *path);
+int pvglu_plan_load(const char *path, pvglu_plan **out, int *P, int *Prow);
+
 /* Kernel schedule: variant and instances-per-chain tile width.
  *   PVGLU_SCHED_LEVEL      one kernel per level (reference launch structure,
  *                          PV/factorize_electrical_system_CUDA.cu:67-144)

@@ -70,6 +70,9 @@ def lib():
         "pvglu_plan_info": (C.c_int, [_VP, C.c_int]),
         "pvglu_plan_array_len": (C.c_int, [_VP, C.c_int]),
         "pvglu_plan_export": (C.c_int, [_VP, C.c_int, _P_INT]),
+        "pvglu_plan_save": (C.c_int, [_VP, C.c_char_p, _P_INT, _P_INT]),
+        "pvglu_plan_file_n": (C.c_int, [C.c_char_p]),
+        "pvglu_plan_load": (C.c_int, [C.c_char_p, C.POINTER(_VP), _P_INT, _P_INT]),
         "pvglu_set_schedule": (C.c_int, [_VP, C.c_int, C.c_int]),
         "pvglu_get_schedule": (C.c_int, [_VP, _P_INT, _P_INT]),
         "pvglu_set_stream_groups": (C.c_int, [_VP, C.c_int]),
@@ -141,6 +144,27 @@ class Plan:
             _lib.pvglu_plan_destroy(h)
 
     __del__ = close
+
+    def save(self, path, P=None, Prow=None):
+        """Write the plan (and optionally its ordering) to ``path``."""
+        p = None if P is None else _i32(P)
+        r = None if Prow is None else _i32(Prow)
+        _check(lib().pvglu_plan_save(self.handle, os.fsencode(path),
+                                     None if p is None else _ip(p),
+                                     None if r is None else _ip(r)))
+
+    @classmethod
+    def load(cls, path):
+        """Read a plan file: returns (plan, P, Prow)."""
+        n = _check(lib().pvglu_plan_file_n(os.fsencode(path)))
+        P = np.zeros(n, np.int32)
+        Prow = np.zeros(n, np.int32)
+        h = _VP()
+        _check(lib().pvglu_plan_load(os.fsencode(path), C.byref(h), _ip(P), _ip(Prow)))
+        plan = cls.__new__(cls)
+        plan._h = h
+        plan.n = n
+        return plan, P, Prow
 
     @property
     def handle(self):
