@@ -273,6 +273,16 @@ int pvglu_pv_positions(const pvglu_pv *pv, double *x, double *y);
  * the uniform 1000 W/m^2 sky. */
 int pvglu_pv_irradiance(const pvglu_pv *pv, int64_t scenario, double t,
                         double *g_out);
+
+/* Device cloud generator (the analogue of PV/001_clouds.cu:96-155 for the
+ * config cloud model): irradiance of `batch` instances, instance k at
+ * scenarios[k] (< 0: uniform 1000 W/m^2) and time times[k] s, into g_dev
+ * [module][instance] on `stream`.  scenarios and times are host arrays.  The
+ * first call (and a call with a larger batch) allocates the plant's
+ * parameter buffer; the call waits for its own upload before returning. */
+int pvglu_pv_irradiance_device(pvglu_pv *pv, int batch,
+                               const int64_t *scenarios, const double *times,
+                               double *g_dev, void *stream);
 /* MNA pattern of the unpermuted system (CSR, sorted) and AMD + row-swap
  * reordering of it: reorder_matrix (PV/004d_reorder_matrix.cpp:27-47). */
 int pvglu_pv_pattern(const pvglu_pv *pv, int *rowptr, int *colidx);

@@ -267,6 +267,7 @@ _PV_SIG = {
     "pvglu_pv_buses": (C.c_int, [_VP, _VP]),
     "pvglu_pv_positions": (C.c_int, [_VP, _VP, _VP]),
     "pvglu_pv_irradiance": (C.c_int, [_VP, C.c_int64, C.c_double, _VP]),
+    "pvglu_pv_irradiance_device": (C.c_int, [_VP, C.c_int, _VP, _VP, _VP, _VP]),
     "pvglu_pv_pattern": (C.c_int, [_VP, _VP, _VP]),
     "pvglu_pv_module_current": (C.c_int, [C.c_double, C.c_double, C.c_double, _P_DBL, _P_DBL,
                                           _P_DBL]),
@@ -375,6 +376,17 @@ class PV:
         g = np.zeros(self.nmod)
         _check(_pvlib().pvglu_pv_irradiance(self._h, int(scenario), float(t), _dp(g)))
         return g
+
+    def irradiance_device(self, scenarios, times, g_dev, stream=None):
+        """Device cloud generator into g_dev [module][instance] (instance k:
+        scenario scenarios[k] at time times[k] s)."""
+        sc = np.ascontiguousarray(scenarios, np.int64)
+        tt = np.ascontiguousarray(times, np.float64)
+        if sc.shape != tt.shape:
+            raise ValueError("scenarios and times differ in length")
+        _check(_pvlib().pvglu_pv_irradiance_device(
+            self._h, len(sc), sc.ctypes.data_as(_VP), tt.ctypes.data_as(_VP), _ptr(g_dev),
+            _stream(stream)))
 
     def pattern(self):
         rp = np.zeros(self.n + 1, np.int32)

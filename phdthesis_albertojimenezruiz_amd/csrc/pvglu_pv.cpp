@@ -163,6 +163,7 @@ extern "C" void pvglu_pv_destroy(pvglu_pv *pv) {
     return;
   newton_dev_destroy(pv->nd);
   dropin_dev_destroy(pv->dd);
+  cloud_dev_destroy(pv->cd);
   pvglu_plan_destroy(pv->plan);
   delete pv;
 }
@@ -224,6 +225,25 @@ extern "C" int pvglu_pv_positions(const pvglu_pv *pv, double *x, double *y) {
   return PVGLU_OK;
 }
 
+namespace pvglu {
+// Seeded cloud field of one scenario: 8 Gaussian shadow blobs over a
+// 1000 W/m^2 clear sky, sigma 50..300 m, depth 0.2..0.8, drift 5..15 m/s at a
+// uniform heading.  prm = [cx(8) cy(8) vx(8) vy(8) sigma(8) depth(8)].
+void cloud_params(const pvglu_pv *pv, int64_t scenario, double *prm) {
+  std::mt19937_64 rng(pv->seed + 1 + (uint64_t)scenario);
+  for (int k = 0; k < 8; k++) {
+    prm[k] = 1000.0 * uniform01(rng());
+    prm[8 + k] = 1000.0 * uniform01(rng());
+    prm[32 + k] = 50.0 + 250.0 * uniform01(rng());
+    prm[40 + k] = 0.2 + 0.6 * uniform01(rng());
+    const double sp = 5.0 + 10.0 * uniform01(rng());
+    const double an = 2.0 * M_PI * uniform01(rng());
+    prm[16 + k] = sp * std::cos(an);
+    prm[24 + k] = sp * std::sin(an);
+  }
+}
+} // namespace pvglu
+
 extern "C" int pvglu_pv_irradiance(const pvglu_pv *pv, int64_t scenario,
                                    double t, double *g_out) {
   if (!pv || !g_out)
@@ -233,20 +253,10 @@ extern "C" int pvglu_pv_irradiance(const pvglu_pv *pv, int64_t scenario,
       g_out[m] = 1000.0;
     return PVGLU_OK;
   }
-  // Seeded cloud field: 8 Gaussian shadow blobs over a 1000 W/m^2 clear sky,
-  // sigma 50..300 m, depth 0.2..0.8, drift 5..15 m/s at a uniform heading.
-  std::mt19937_64 rng(pv->seed + 1 + (uint64_t)scenario);
-  double cx[8], cy[8], vx[8], vy[8], sg[8], dp[8];
-  for (int k = 0; k < 8; k++) {
-    cx[k] = 1000.0 * uniform01(rng());
-    cy[k] = 1000.0 * uniform01(rng());
-    sg[k] = 50.0 + 250.0 * uniform01(rng());
-    dp[k] = 0.2 + 0.6 * uniform01(rng());
-    const double sp = 5.0 + 10.0 * uniform01(rng());
-    const double an = 2.0 * M_PI * uniform01(rng());
-    vx[k] = sp * std::cos(an);
-    vy[k] = sp * std::sin(an);
-  }
+  double prm[kCloudParams];
+  cloud_params(pv, scenario, prm);
+  const double *cx = prm, *cy = prm + 8, *vx = prm + 16, *vy = prm + 24,
+               *sg = prm + 32, *dp = prm + 40;
   for (int m = 0; m < pv->nmod; m++) {
     double shade = 0.0;
     for (int k = 0; k < 8; k++) {

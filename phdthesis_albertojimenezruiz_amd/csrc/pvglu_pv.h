@@ -17,6 +17,10 @@ void dropin_dev_destroy(DropinDev *dd);
 // std::mt19937_64-based uniform in [0,1) with a fixed 53-bit construction so
 // draws do not depend on the standard library's distribution code.
 double uniform01(uint64_t word);
+
+struct CloudDev; // device buffers of the cloud generator (pvglu_clouds.hip)
+void cloud_dev_destroy(CloudDev *cd);
+constexpr int kCloudParams = 48;
 } // namespace pvglu
 
 struct pvglu_pv {
@@ -38,8 +42,10 @@ struct pvglu_pv {
 
   pvglu::NewtonDev *nd = nullptr;
   pvglu::DropinDev *dd = nullptr;
+  pvglu::CloudDev *cd = nullptr;
 };
 
 namespace pvglu {
 int pv_prepare(pvglu_pv *pv); // symbolic setup (host reorder + GPU fill + plan)
+void cloud_params(const pvglu_pv *pv, int64_t scenario, double *prm);
 }

@@ -254,3 +254,24 @@ def test_batched_iterations_track_oracle(cfg0):
 
 def _t(a, dev):
     return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+def test_device_cloud_generator_matches_host(dev):
+    """pvglu_pv_irradiance_device against the host generator (same seeded blob
+    draws; exp() may differ in the last bits): 1e-12 relative, clear sky
+    exactly, several scenarios and times in one batch."""
+    plant = pv.PV(10, 24, 28, seed=7)
+    scen = [-1, 0, 3, 3, 11, 250]
+    times = [0.0, 0.0, 0.0, 17.5, 3.0, 60.0]
+    g = torch.empty((plant.nmod, len(scen)), dtype=torch.float64, device=dev)
+    plant.irradiance_device(scen, times, g)
+    gd = g.cpu().numpy()
+    for k, (s_, t_) in enumerate(zip(scen, times)):
+        want = plant.irradiance(s_, t_)
+        assert np.allclose(gd[:, k], want, rtol=1e-12, atol=1e-9), k
+    assert np.all(gd[:, 0] == 1000.0)
+    assert not np.allclose(gd[:, 2], gd[:, 3])           # the clouds moved
+    # a larger batch regrows the parameter buffer
+    g2 = torch.empty((plant.nmod, 40), dtype=torch.float64, device=dev)
+    plant.irradiance_device(list(range(40)), [1.0] * 40, g2)
+    assert np.allclose(g2[:, 39].cpu().numpy(), plant.irradiance(39, 1.0), rtol=1e-12, atol=1e-9)
