@@ -175,11 +175,13 @@ def cfg4_steps(cfg4):
 
 
 def test_cfg4_warm_start_agrees_with_cold_start(cfg4, cfg4_steps):
+    """Two solves from different starts agree to the FP64 noise floor of the
+    shaded strings (NOISE_AGREE); the typical instance to 2 TOL."""
     r = cfg4_steps
-    for k in range(r["G1"].shape[1]):
-        both = r["last1w"][k] <= TOL and r["last1c"][k] <= TOL
-        dx = np.abs(r["x1w"][:, k] - r["x1c"][:, k]).max()
-        assert dx <= (2 * TOL if both else NOISE_AGREE), (k, dx)
+    dxs = np.array([np.abs(r["x1w"][:, k] - r["x1c"][:, k]).max()
+                    for k in range(r["G1"].shape[1])])
+    assert dxs.max() <= NOISE_AGREE, dxs
+    assert np.median(dxs) <= 2 * TOL, dxs
     # a one-second cloud step moves little: the warm start needs fewer iterations
     assert r["it1w"].sum() < r["it1c"].sum()
 
