@@ -9,6 +9,7 @@
 // in the last bits of exp().
 #include <hip/hip_runtime.h>
 
+#include "pvglu_module.h"
 #include "pvglu_pv.h"
 
 #include <vector>
@@ -38,10 +39,8 @@ constexpr int kStride = kCloudParams + 2; // params, time, clear-sky flag
 __global__ void __launch_bounds__(256)
     k_clouds(int nmod, int B, const double *__restrict__ pos,
              const double *__restrict__ prm, double *__restrict__ g) {
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long)nmod * B)
-    return;
-  const int m = (int)(gid / B), inst = (int)(gid % B);
+  PVGLU_ROW_INDEX(B);
+  const int m = row_;
   const double *p = prm + (size_t)inst * kStride;
   if (p[kCloudParams + 1] != 0.0) { // uniform 1000 W/m^2
     g[gid] = 1000.0;
@@ -108,10 +107,9 @@ extern "C" int pvglu_pv_irradiance_device(pvglu_pv *pv, int batch,
   HIP_TRY(hipMemcpyAsync(cd->prm, cd->host.data(),
                          sizeof(double) * cd->host.size(),
                          hipMemcpyHostToDevice, s));
-  const long total = (long)pv->nmod * batch;
-  hipLaunchKernelGGL(k_clouds, dim3((unsigned)((total + 255) / 256)),
-                     dim3(256), 0, s, pv->nmod, batch, cd->pos, cd->prm,
-                     g_dev);
+  const RowGrid rg = row_grid(pv->nmod, batch);
+  hipLaunchKernelGGL(k_clouds, rg.grid, rg.block, 0, s, pv->nmod, batch,
+                     cd->pos, cd->prm, g_dev);
   HIP_TRY(hipGetLastError());
   // the staging vector is reused by the next call: finish the copy first
   HIP_TRY(hipStreamSynchronize(s));

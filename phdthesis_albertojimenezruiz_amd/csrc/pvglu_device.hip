@@ -918,7 +918,9 @@ __global__ void k_fill_int(int *p, int n, int v) {
 }
 
 // ----------------------------------------------------------- 011 update
-// One thread per (element chunk, instance).  Per element the damping factor
+// One thread per (element chunk, instance); Σ|Δ| is accumulated with atomics
+// (order-dependent in the last bits; the maximum is exact).  Per element the
+// damping factor
 // halves until |candidate - old| <= 1 V and nothing is NaN
 // (is_value_change_justified, PV/011_calculate_new_iteration.cpp:6-17); the
 // reference's loop halves each unjustified element's factor independently and
@@ -929,12 +931,11 @@ __global__ void __launch_bounds__(256)
                     double *__restrict__ v, double *delta_sum,
                     unsigned long long *delta_max, const int *active,
                     int *status) {
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long total = (long)((ne + per_thread - 1) / per_thread) * B;
-  if (gid >= total)
+  // block -> (element chunk, tile of up to 256 instances)
+  const int chunk = (int)blockIdx.x;
+  const int in = (int)(blockIdx.y * blockDim.x + threadIdx.x);
+  if (in >= B)
     return;
-  const int in = (int)(gid % B);
-  const int chunk = (int)(gid / B);
   if (active && !active[in])
     return; // converged or stalled instance: frozen
   const int e0 = chunk * per_thread;
@@ -1505,14 +1506,15 @@ extern "C" int pvglu_newton_update(pvglu_plan *P, int ne, int batch,
       !delta_max)
     return fail(PVGLU_E_ARG, "newton_update: bad arguments");
   const int per = 32;
-  const long total = (long)((ne + per - 1) / per) * batch;
-  const int blocks = (int)((total + 255) / 256);
+  const int chunks = (ne + per - 1) / per;
+  const int bs = batch >= 256 ? 256 : ((batch + 63) / 64) * 64;
   // zero the per-instance reductions (fill kernel: capture-safe)
   hipLaunchKernelGGL(k_fill_int, dim3((2 * batch + 255) / 256), dim3(256), 0,
                      (hipStream_t)stream, (int *)delta_sum, 2 * batch, 0);
   hipLaunchKernelGGL(k_fill_int, dim3((2 * batch + 255) / 256), dim3(256), 0,
                      (hipStream_t)stream, (int *)delta_max, 2 * batch, 0);
-  hipLaunchKernelGGL(k_newton_update, dim3(blocks), dim3(256), 0,
+  hipLaunchKernelGGL(k_newton_update, dim3(chunks, (batch + bs - 1) / bs),
+                     dim3(bs), 0,
                      (hipStream_t)stream, ne, batch, per, pos, neg, x, v_elem,
                      delta_sum, (unsigned long long *)delta_max, active,
                      P->dev->status);

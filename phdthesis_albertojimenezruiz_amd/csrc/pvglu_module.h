@@ -65,6 +65,26 @@ PVGLU_HD double lambertw_exp(double x) {
   return w;
 }
 
+#ifdef __HIPCC__
+// Launch shape of the [row][instance] kernels: one block covers up to 256
+// instances of one row (grid.x = rows, grid.y = instance tiles), so a
+// thread's row and instance come from the block indices without a 64-bit
+// division and the row's metadata is uniform in the block.
+struct RowGrid {
+  dim3 grid, block;
+};
+inline RowGrid row_grid(long rows, int B) {
+  const int bs = B >= 256 ? 256 : ((B + 63) / 64) * 64;
+  return {dim3((unsigned)rows, (unsigned)((B + bs - 1) / bs)), dim3(bs)};
+}
+#define PVGLU_ROW_INDEX(B)                                                     \
+  const int row_ = (int)blockIdx.x;                                            \
+  const int inst = (int)(blockIdx.y * blockDim.x + threadIdx.x);               \
+  if (inst >= (B))                                                             \
+    return;                                                                    \
+  const size_t gid = (size_t)row_ * (size_t)(B) + (size_t)inst
+#endif
+
 struct ModuleEval {
   double I, dIdV, dIdIph;
 };

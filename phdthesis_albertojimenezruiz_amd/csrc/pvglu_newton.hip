@@ -107,12 +107,10 @@ __global__ void k_eval(int nm, int B, const double *__restrict__ Girr,
                        const double *__restrict__ rsh,
                        const double *__restrict__ v, double *__restrict__ geq,
                        double *__restrict__ ieq, const int *active) {
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long)nm * B)
-    return;
-  if (active && !active[gid % B])
+  PVGLU_ROW_INDEX(B);
+  if (active && !active[inst])
     return; // frozen instance: its system (and so its solution) stays as is
-  const int e = (int)(gid / B);
+  const int e = row_;
   const double V = v[gid];
   const ModuleEval me = module_current(V, kIphPerIrradiance * Girr[gid], rsh[e]);
   const double f = -me.I, g = -me.dIdV;
@@ -129,10 +127,8 @@ __global__ void k_stamp_a(int nnz, int B, const double *__restrict__ base,
                           const double *__restrict__ geq,
                           const double *__restrict__ gvp, double *A,
                           const int *active) {
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long)nnz * B)
-    return;
-  const int q = (int)(gid / B), inst = (int)(gid % B);
+  PVGLU_ROW_INDEX(B);
+  const int q = row_;
   if (active && !active[inst])
     return;
   const double gv = gvp[inst]; // homotopy conductance of this instance's stage
@@ -153,10 +149,8 @@ __global__ void k_stamp_b(int n, int B, const double *__restrict__ base,
                           const int *__restrict__ sgn,
                           const double *__restrict__ ieq, double *b,
                           const int *active) {
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long)n * B)
-    return;
-  const int r = (int)(gid / B), inst = (int)(gid % B);
+  PVGLU_ROW_INDEX(B);
+  const int r = row_;
   if (active && !active[inst])
     return;
   double v = base[r];
@@ -175,10 +169,8 @@ __global__ void k_zero(long count, double *p) {
 
 __global__ void k_power_grad_bus(int nbus, int B, const int *__restrict__ bus_c,
                                  const double *__restrict__ x, double *gP) {
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long)nbus * B)
-    return;
-  const int k = (int)(gid / B), inst = (int)(gid % B);
+  PVGLU_ROW_INDEX(B);
+  const int k = row_;
   const size_t idx = (size_t)bus_c[k] * B + inst;
   gP[idx] = kNortonG * (2.0 * x[idx] - kNortonV);
 }
@@ -190,10 +182,8 @@ __global__ void k_grad_modules(int nm, int B, const double *__restrict__ Girr,
                                const int *__restrict__ rpos,
                                const int *__restrict__ rneg,
                                const double *__restrict__ mu, double *dG) {
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long)nm * B)
-    return;
-  const int e = (int)(gid / B), inst = (int)(gid % B);
+  PVGLU_ROW_INDEX(B);
+  const int e = row_;
   const ModuleEval me =
       module_current(v[gid], kIphPerIrradiance * Girr[gid], rsh[e]);
   const double lp = rpos[e] >= 0 ? mu[(size_t)rpos[e] * B + inst] : 0.0;
@@ -208,10 +198,8 @@ __global__ void k_grad_cables(int nc, int B, const int *__restrict__ ra,
                               const int *__restrict__ cb,
                               const double *__restrict__ x,
                               const double *__restrict__ mu, double *dg) {
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= (long)nc * B)
-    return;
-  const int c = (int)(gid / B), inst = (int)(gid % B);
+  PVGLU_ROW_INDEX(B);
+  const int c = row_;
   const double dv = x[(size_t)ca[c] * B + inst] - x[(size_t)cb[c] * B + inst];
   const double dl = mu[(size_t)ra[c] * B + inst] - mu[(size_t)rb[c] * B + inst];
   dg[gid] = -dl * dv;
@@ -468,14 +456,13 @@ inline dim3 grid_for(long count) { return dim3((unsigned)((count + 255) / 256));
 int launch_stamp(pvglu_pv *pv, int B, const int *active) {
   NewtonDev *d = pv->nd;
   const int nm = pv->nmod, n = pv->n, nnz = pv->plan->nnz;
-  hipLaunchKernelGGL(k_eval, grid_for((long)nm * B), dim3(256), 0, d->s, nm, B,
-                     d->G, d->rsh, d->v, d->geq, d->ieq, active);
-  hipLaunchKernelGGL(k_stamp_a, grid_for((long)nnz * B), dim3(256), 0, d->s,
-                     nnz, B, d->base_a, d->a_ptr, d->a_mod, d->a_sgn, d->geq,
-                     d->gvs, d->A, active);
-  hipLaunchKernelGGL(k_stamp_b, grid_for((long)n * B), dim3(256), 0, d->s, n,
-                     B, d->base_b, d->b_ptr, d->b_mod, d->b_sgn, d->ieq, d->b,
-                     active);
+  const RowGrid ge = row_grid(nm, B), ga = row_grid(nnz, B), gb = row_grid(n, B);
+  hipLaunchKernelGGL(k_eval, ge.grid, ge.block, 0, d->s, nm, B, d->G, d->rsh,
+                     d->v, d->geq, d->ieq, active);
+  hipLaunchKernelGGL(k_stamp_a, ga.grid, ga.block, 0, d->s, nnz, B, d->base_a,
+                     d->a_ptr, d->a_mod, d->a_sgn, d->geq, d->gvs, d->A, active);
+  hipLaunchKernelGGL(k_stamp_b, gb.grid, gb.block, 0, d->s, n, B, d->base_b,
+                     d->b_ptr, d->b_mod, d->b_sgn, d->ieq, d->b, active);
   HIP_TRY(hipGetLastError());
   return PVGLU_OK;
 }
@@ -661,16 +648,17 @@ extern "C" int pvglu_pv_gradients(pvglu_pv *pv, int batch, const double *g,
     return st;
   hipLaunchKernelGGL(k_zero, grid_for((long)n * B), dim3(256), 0, d->s,
                      (long)n * B, d->gP);
-  hipLaunchKernelGGL(k_power_grad_bus, grid_for((long)pv->I * B), dim3(256), 0,
-                     d->s, pv->I, B, d->bus_c, d->x, d->gP);
+  const RowGrid gbus = row_grid(pv->I, B), gm = row_grid(nm, B),
+                gc = row_grid(nc, B);
+  hipLaunchKernelGGL(k_power_grad_bus, gbus.grid, gbus.block, 0, d->s, pv->I, B,
+                     d->bus_c, d->x, d->gP);
   if ((st = pvglu_solve_transpose(pv->plan, d->LU, d->gP, d->mu, B, d->s)))
     return st;
-  hipLaunchKernelGGL(k_grad_modules, grid_for((long)nm * B), dim3(256), 0,
-                     d->s, nm, B, d->G, d->rsh, d->v, d->rpos, d->rneg, d->mu,
-                     d->dG);
-  hipLaunchKernelGGL(k_grad_cables, grid_for((long)nc * B), dim3(256), 0, d->s,
-                     nc, B, d->cab_ra, d->cab_rb, d->cab_ca, d->cab_cb, d->x,
-                     d->mu, d->dgc);
+  hipLaunchKernelGGL(k_grad_modules, gm.grid, gm.block, 0, d->s, nm, B, d->G,
+                     d->rsh, d->v, d->rpos, d->rneg, d->mu, d->dG);
+  hipLaunchKernelGGL(k_grad_cables, gc.grid, gc.block, 0, d->s, nc, B,
+                     d->cab_ra, d->cab_rb, d->cab_ca, d->cab_cb, d->x, d->mu,
+                     d->dgc);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(d->s));
   int code = 0, col = 0, inst = 0;
@@ -848,13 +836,13 @@ extern "C" int pvglu_pv_stamp(pvglu_pv *pv, int batch, const double *g,
   }
   hipLaunchKernelGGL(k_fill_double, grid_for(B), dim3(256), 0, s, d->gvs, B,
                      g_virtual);
-  hipLaunchKernelGGL(k_eval, grid_for((long)nm * B), dim3(256), 0, s, nm, B, g,
-                     d->rsh, v, geq, ieq, nullptr);
-  hipLaunchKernelGGL(k_stamp_a, grid_for((long)nnz * B), dim3(256), 0, s, nnz,
-                     B, d->base_a, d->a_ptr, d->a_mod, d->a_sgn, geq, d->gvs, a,
-                     nullptr);
-  hipLaunchKernelGGL(k_stamp_b, grid_for((long)n * B), dim3(256), 0, s, n, B,
-                     d->base_b, d->b_ptr, d->b_mod, d->b_sgn, ieq, b, nullptr);
+  const RowGrid ge = row_grid(nm, B), ga = row_grid(nnz, B), gb = row_grid(n, B);
+  hipLaunchKernelGGL(k_eval, ge.grid, ge.block, 0, s, nm, B, g, d->rsh, v, geq,
+                     ieq, nullptr);
+  hipLaunchKernelGGL(k_stamp_a, ga.grid, ga.block, 0, s, nnz, B, d->base_a,
+                     d->a_ptr, d->a_mod, d->a_sgn, geq, d->gvs, a, nullptr);
+  hipLaunchKernelGGL(k_stamp_b, gb.grid, gb.block, 0, s, n, B, d->base_b,
+                     d->b_ptr, d->b_mod, d->b_sgn, ieq, b, nullptr);
   HIP_TRY(hipGetLastError());
   return PVGLU_OK;
 }
