@@ -64,7 +64,9 @@ struct DevicePlan {
   int *lu_rec = nullptr, *lu_seg = nullptr, *lu_run = nullptr,
       *lu_fast = nullptr, *u_rec = nullptr, *u_seg = nullptr, *u_run = nullptr,
       *u_fast = nullptr, *l_rec = nullptr, *l_seg = nullptr, *l_run = nullptr,
-      *l_fast = nullptr;
+      *l_fast = nullptr, *ut_rec = nullptr, *ut_seg = nullptr, *ut_run = nullptr,
+      *ut_fast = nullptr, *lt_rec = nullptr, *lt_seg = nullptr,
+      *lt_run = nullptr, *lt_fast = nullptr;
   int *status = nullptr; // [code, column, instance, pad]
   std::vector<void *> allocs;
   // component groups (first component of each group, 1 / 2 / 4 groups) and
@@ -103,6 +105,14 @@ struct Pat {
   const int *__restrict__ l_seg;
   const int *__restrict__ l_run;
   const int *__restrict__ l_fast;
+  const int *__restrict__ ut_rec;
+  const int *__restrict__ ut_seg;
+  const int *__restrict__ ut_run;
+  const int *__restrict__ ut_fast;
+  const int *__restrict__ lt_rec;
+  const int *__restrict__ lt_seg;
+  const int *__restrict__ lt_run;
+  const int *__restrict__ lt_fast;
   int *status;
 };
 
@@ -121,7 +131,9 @@ static Pat pat_of(const DevicePlan *d) {
              d->diag_slot, d->lrow_ptr, d->lrow_slot, d->lrow_col, d->urow_ptr,
              d->urow_slot, d->urow_col, d->lu_rec,  d->lu_seg,    d->lu_run,
              d->lu_fast,   d->u_rec,    d->u_seg,  d->u_run,     d->u_fast,
-             d->l_rec,     d->l_seg,    d->l_run,  d->l_fast,    d->status};
+             d->l_rec,     d->l_seg,    d->l_run,  d->l_fast,    d->ut_rec,
+             d->ut_seg,    d->ut_run,   d->ut_fast, d->lt_rec,   d->lt_seg,
+             d->lt_run,    d->lt_fast,  d->status};
 }
 static Lev lev_of(const DevLevels &L) {
   return Lev{L.level_chain_ptr, L.chain_col_ptr, L.cols,
@@ -711,26 +723,41 @@ __device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
     }
     return;
   }
-  if constexpr (OP == OP_USOLVE || OP == OP_LSOLVE) {
-    constexpr bool DIV = OP == OP_USOLVE;
-    const int *rec = DIV ? P.u_rec : P.l_rec;
-    const int *seg = DIV ? P.u_seg : P.l_seg;
-    const double *src = DIV ? x : b;
-    if (uniform<W>((DIV ? P.u_fast : P.l_fast)[chain])) {
+  if constexpr (OP == OP_USOLVE || OP == OP_LSOLVE || OP == OP_UT ||
+                OP == OP_LT) {
+    // row programs: backward (U), forward (L), adjoint U^T (divides) and L^T
+    constexpr bool DIV = OP == OP_USOLVE || OP == OP_UT;
+    const int *rec, *seg, *run, *whole;
+    if constexpr (OP == OP_USOLVE) {
+      rec = P.u_rec, seg = P.u_seg, run = P.u_run, whole = P.u_fast;
+    } else if constexpr (OP == OP_LSOLVE) {
+      rec = P.l_rec, seg = P.l_seg, run = P.l_run, whole = P.l_fast;
+    } else if constexpr (OP == OP_UT) {
+      rec = P.ut_rec, seg = P.ut_seg, run = P.ut_run, whole = P.ut_fast;
+    } else {
+      rec = P.lt_rec, seg = P.lt_seg, run = P.lt_run, whole = P.lt_fast;
+    }
+    // source of the right-hand side: forward b, U^T g; in place otherwise
+    const double *src = (OP == OP_LSOLVE || OP == OP_UT) ? b : x;
+    if (uniform<W>(whole[chain])) {
       fast_row_chain<W, DIV>(rec, seg, t0, t1, LU, src, x, B, inst);
       return;
     }
-    const int *run = DIV ? P.u_run : P.l_run;
     for (int t = t0; t < t1;) {
       const int e = uniform<W>(run[t]);
       if (e > t) {
         fast_row_chain<W, DIV>(rec, seg, t, e, LU, src, x, B, inst);
         t = e;
       } else {
-        if constexpr (DIV)
-          usolve_row<W>(P, uniform<W>(L.cols[t]), LU, x, B, inst);
+        const int c = uniform<W>(L.cols[t]);
+        if constexpr (OP == OP_USOLVE)
+          usolve_row<W>(P, c, LU, x, B, inst);
+        else if constexpr (OP == OP_LSOLVE)
+          lsolve_row<W>(P, c, LU, b, x, B, inst);
+        else if constexpr (OP == OP_UT)
+          utsolve_col<W>(P, c, LU, b, x, B, inst);
         else
-          lsolve_row<W>(P, uniform<W>(L.cols[t]), LU, b, x, B, inst);
+          ltsolve_col<W>(P, c, LU, x, B, inst);
         ++t;
       }
     }
@@ -1163,6 +1190,14 @@ extern "C" int pvglu_plan_upload(pvglu_plan *P) {
       (st = dev_upload(d, &d->l_seg, P->fast_l_seg)) ||
       (st = dev_upload(d, &d->l_run, P->fast_l_run)) ||
       (st = dev_upload(d, &d->l_fast, P->fast_l_chain)) ||
+      (st = dev_upload(d, &d->ut_rec, P->fast_ut_rec)) ||
+      (st = dev_upload(d, &d->ut_seg, P->fast_ut_seg)) ||
+      (st = dev_upload(d, &d->ut_run, P->fast_ut_run)) ||
+      (st = dev_upload(d, &d->ut_fast, P->fast_ut_chain)) ||
+      (st = dev_upload(d, &d->lt_rec, P->fast_lt_rec)) ||
+      (st = dev_upload(d, &d->lt_seg, P->fast_lt_seg)) ||
+      (st = dev_upload(d, &d->lt_run, P->fast_lt_run)) ||
+      (st = dev_upload(d, &d->lt_fast, P->fast_lt_chain)) ||
       (st = dev_upload(d, &d->status, std::vector<int>(4, 0))) ||
       (st = setup_groups(P))) {
     device_plan_destroy(P);

@@ -76,13 +76,17 @@ struct pvglu_plan {
   pvglu::KernelLevels k_lu, k_l, k_u, k_lt;
   // Register-forwarding programs (see compile_fast_programs): one 16-int
   // record per refactor column and one 8-int record per backward-substitution
-  // row (and one per forward-substitution row of the L levels), in kernel
+  // row (and one per forward-substitution row of the L levels and per column
+  // of the transposed sweeps), in kernel
   // order, the end of the register run holding each position
   // (0: generic column) and a per-chain whole-chain flag.
   std::vector<int> fast_lu_rec, fast_lu_run, fast_lu_chain, fast_lu_seg;
   std::vector<int> fast_u_rec, fast_u_run, fast_u_chain, fast_u_seg;
   std::vector<int> fast_l_rec, fast_l_run, fast_l_chain, fast_l_seg;
+  std::vector<int> fast_ut_rec, fast_ut_run, fast_ut_chain, fast_ut_seg;
+  std::vector<int> fast_lt_rec, fast_lt_run, fast_lt_chain, fast_lt_seg;
   int fast_lu_chains = 0, fast_u_chains = 0, fast_l_chains = 0;
+  int fast_ut_chains = 0, fast_lt_chains = 0;
   int fast_lu_segs = 0, fast_u_segs = 0;
   uint64_t hash = 0;
 

@@ -302,6 +302,22 @@ static void encode_l_record(const pvglu_plan *P, int i, int prev, int *r) {
   }
 }
 
+// Transposed sweeps (adjoint): column j of U above its diagonal (U^T, with
+// the division) or of L below it (L^T), as row records over CSC slots.
+static void encode_t_record(const pvglu_plan *P, int j, int prev, bool upper,
+                            int *r) {
+  const int d = P->diag_slot[j];
+  const int q0 = upper ? P->cp[j] : d + 1, q1 = upper ? d : P->cp[j + 1];
+  r[0] = j;
+  r[1] = d;
+  r[2] = q1 - q0;
+  for (int k = 0; k < q1 - q0; k++) {
+    const int i = P->ri[q0 + k];
+    r[4 + 2 * k] = q0 + k - d;
+    r[5 + 2 * k] = (i == prev) ? -1 : i;
+  }
+}
+
 static void encode_u_record(const pvglu_plan *P, int i, int prev, int *r) {
   const int nt = P->urow_ptr[i + 1] - P->urow_ptr[i];
   r[0] = i;
@@ -433,6 +449,20 @@ void compile_fast_programs(pvglu_plan *P) {
       [&](int i, int prev, int *r) { encode_l_record(P, i, prev, r); },
       P->fast_l_rec, P->fast_l_run, P->fast_l_chain);
   compile_segments(P->fast_l_rec, 8, P->fast_l_run, P->n, P->fast_l_seg,
+                   row_forwarded);
+  P->fast_ut_chains = compile_runs(
+      P->k_lu, P->n, 8,
+      [&](int j) { return P->diag_slot[j] - P->cp[j] <= kFastMaxLr; },
+      [&](int j, int prev, int *r) { encode_t_record(P, j, prev, true, r); },
+      P->fast_ut_rec, P->fast_ut_run, P->fast_ut_chain);
+  compile_segments(P->fast_ut_rec, 8, P->fast_ut_run, P->n, P->fast_ut_seg,
+                   row_forwarded);
+  P->fast_lt_chains = compile_runs(
+      P->k_lt, P->n, 8,
+      [&](int j) { return P->cp[j + 1] - P->diag_slot[j] - 1 <= kFastMaxLr; },
+      [&](int j, int prev, int *r) { encode_t_record(P, j, prev, false, r); },
+      P->fast_lt_rec, P->fast_lt_run, P->fast_lt_chain);
+  compile_segments(P->fast_lt_rec, 8, P->fast_lt_run, P->n, P->fast_lt_seg,
                    row_forwarded);
   P->fast_lu_segs = P->fast_u_segs = 0;
   for (int t = 0; t < P->n; t++) {
