@@ -66,7 +66,7 @@ struct DevicePlan {
       *u_fast = nullptr, *l_rec = nullptr, *l_seg = nullptr, *l_run = nullptr,
       *l_fast = nullptr, *ut_rec = nullptr, *ut_seg = nullptr, *ut_run = nullptr,
       *ut_fast = nullptr, *lt_rec = nullptr, *lt_seg = nullptr,
-      *lt_run = nullptr, *lt_fast = nullptr;
+      *lt_run = nullptr, *lt_fast = nullptr, *drf = nullptr;
   int *status = nullptr; // [code, column, instance, pad]
   std::vector<void *> allocs;
   // component groups (first component of each group, 1 / 2 / 4 groups) and
@@ -113,6 +113,7 @@ struct Pat {
   const int *__restrict__ lt_seg;
   const int *__restrict__ lt_run;
   const int *__restrict__ lt_fast;
+  const int *__restrict__ drf;
   int *status;
 };
 
@@ -133,7 +134,7 @@ static Pat pat_of(const DevicePlan *d) {
              d->lu_fast,   d->u_rec,    d->u_seg,  d->u_run,     d->u_fast,
              d->l_rec,     d->l_seg,    d->l_run,  d->l_fast,    d->ut_rec,
              d->ut_seg,    d->ut_run,   d->ut_fast, d->lt_rec,   d->lt_seg,
-             d->lt_run,    d->lt_fast,  d->status};
+             d->lt_run,    d->lt_fast,  d->drf,    d->status};
 }
 static Lev lev_of(const DevLevels &L) {
   return Lev{L.level_chain_ptr, L.chain_col_ptr, L.cols,
@@ -202,6 +203,39 @@ __device__ __forceinline__ double dot_sub(double v, int k0, int k1, GET get) {
   return v;
 }
 
+// Two sums sharing their first operand: v - sum_k l_k r_k and
+// y - sum_k l_k s_k, each in ascending k (the same operation order as two
+// separate dot_sub calls), from one set of loads.
+template <int CH, class GET>
+__device__ __forceinline__ void chunk_sub2(double &v, double &y, int kb, int k1,
+                                           GET get) {
+  double l[CH], r[CH], t[CH];
+#pragma unroll
+  for (int j = 0; j < CH; ++j)
+    get(min(kb + j, k1 - 1), l[j], r[j], t[j]);
+#pragma unroll
+  for (int j = 0; j < CH; ++j)
+    if (kb + j < k1) {
+      v = v - l[j] * r[j];
+      y = y - l[j] * t[j];
+    }
+}
+
+template <class GET>
+__device__ __forceinline__ void dot_sub2(double &v, double &y, int k0, int k1,
+                                         GET get) {
+  int kb = k0;
+  for (; k1 - kb >= kChunk; kb += kChunk)
+    chunk_sub2<kChunk>(v, y, kb, k1, get);
+  const int rest = k1 - kb;
+  if (rest > kChunk / 2)
+    chunk_sub2<kChunk>(v, y, kb, k1, get);
+  else if (rest > 2)
+    chunk_sub2<kChunk / 2>(v, y, kb, k1, get);
+  else if (rest > 0)
+    chunk_sub2<2>(v, y, kb, k1, get);
+}
+
 template <int W>
 __device__ __forceinline__ double dot_sub_chunked(double v, const int *sl_arr,
                                                   const int *sr_arr,
@@ -235,23 +269,41 @@ __device__ __forceinline__ void refactor_column(const Pat &P, int c,
   const int q1 = uniform<W>(P.cp[c + 1]);
   const int qd = uniform<W>(P.diag[c]);
   double piv = 1.0;
+  bool row_with_diag = false; // forward row computed with the diagonal
+  double yrow = 0.0;
+  if constexpr (FUSE)
+    row_with_diag = uniform<W>(P.drf[c]) != 0;
   // the A values of CH entries are loaded together, then the entries are
   // updated in slot order
   auto entries = [&](auto ch, int qb) {
     constexpr int CH = decltype(ch)::value;
     const int cnt = min(CH, q1 - qb);
     double a[CH];
+    int optr[CH + 1]; // the chunk's update ranges, loaded together
 #pragma unroll
     for (int k = 0; k < CH; ++k)
       a[k] = A[(size_t)min(qb + k, q1 - 1) * B + inst];
+#pragma unroll
+    for (int k = 0; k <= CH; ++k)
+      optr[k] = uniform<W>(P.op_ptr[min(qb + k, q1)]);
 #pragma unroll
     for (int k = 0; k < CH; ++k) {
       if (k >= cnt)
         break;
       const int q = qb + k;
-      double v = dot_sub_chunked<W>(a[k], P.op_l, P.op_u, LU, LU,
-                                    uniform<W>(P.op_ptr[q]),
-                                    uniform<W>(P.op_ptr[q + 1]), B, inst);
+      double v = a[k];
+      const int o0 = optr[k], o1 = optr[k + 1];
+      if (FUSE && row_with_diag && q == qd) {
+        const int r0 = uniform<W>(P.lrow_ptr[c]);
+        yrow = b[(size_t)c * B + inst];
+        dot_sub2(v, yrow, o0, o1, [&](int o, double &l, double &u, double &yj) {
+          l = LU[(size_t)uniform<W>(P.op_l[o]) * B + inst];
+          u = LU[(size_t)uniform<W>(P.op_u[o]) * B + inst];
+          yj = x[(size_t)uniform<W>(P.lrow_col[r0 + (o - o0)]) * B + inst];
+        });
+      } else {
+        v = dot_sub_chunked<W>(v, P.op_l, P.op_u, LU, LU, o0, o1, B, inst);
+      }
       if (q == qd) {
         piv = v;
         if (fabs(v) < 1e-8)
@@ -269,10 +321,12 @@ __device__ __forceinline__ void refactor_column(const Pat &P, int c,
       entries(std::integral_constant<int, kChunk / 2>(), qb);
   }
   if constexpr (FUSE) {
-    const double s = dot_sub_chunked<W>(
-        b[(size_t)c * B + inst], P.lrow_slot, P.lrow_col, LU, x,
-        uniform<W>(P.lrow_ptr[c]), uniform<W>(P.lrow_ptr[c + 1]), B, inst);
-    x[(size_t)c * B + inst] = s;
+    if (row_with_diag)
+      x[(size_t)c * B + inst] = yrow;
+    else
+      x[(size_t)c * B + inst] = dot_sub_chunked<W>(
+          b[(size_t)c * B + inst], P.lrow_slot, P.lrow_col, LU, x,
+          uniform<W>(P.lrow_ptr[c]), uniform<W>(P.lrow_ptr[c + 1]), B, inst);
   }
 }
 
@@ -1198,6 +1252,7 @@ extern "C" int pvglu_plan_upload(pvglu_plan *P) {
       (st = dev_upload(d, &d->lt_seg, P->fast_lt_seg)) ||
       (st = dev_upload(d, &d->lt_run, P->fast_lt_run)) ||
       (st = dev_upload(d, &d->lt_fast, P->fast_lt_chain)) ||
+      (st = dev_upload(d, &d->drf, P->diag_row_fused)) ||
       (st = dev_upload(d, &d->status, std::vector<int>(4, 0))) ||
       (st = setup_groups(P))) {
     device_plan_destroy(P);

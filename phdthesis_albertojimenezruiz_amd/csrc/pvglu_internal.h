@@ -71,6 +71,9 @@ struct pvglu_plan {
   std::vector<int> urow_ptr, urow_slot, urow_col;
   std::vector<int> diag_slot;
   bool fuse_lsolve = false;
+  // per column: the diagonal's updates and row c of the forward sweep share
+  // their L operands (one pass computes both sums)
+  std::vector<int> diag_row_fused;
   int ncomp = 0;
   std::vector<int> comp_of; // component id per column
   pvglu::KernelLevels k_lu, k_l, k_u, k_lt;

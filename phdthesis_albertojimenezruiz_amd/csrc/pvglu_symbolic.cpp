@@ -656,6 +656,20 @@ extern "C" int pvglu_plan_create(int n, const int *rowptr, const int *colidx,
     }
     P->fuse_lsolve = ok;
   }
+  // Columns whose diagonal updates and forward-substitution row use the same
+  // L operands in the same order compute both sums in one pass.
+  P->diag_row_fused.assign(n, 0);
+  if (P->fuse_lsolve)
+    for (int c = 0; c < n; c++) {
+      const int qd = P->diag_slot[c];
+      const int o0 = P->op_ptr[qd], o1 = P->op_ptr[qd + 1];
+      const int r0 = P->lrow_ptr[c], r1 = P->lrow_ptr[c + 1];
+      bool same = o1 - o0 == r1 - r0;
+      for (int k = 0; same && k < o1 - o0; k++)
+        same = P->op_l[o0 + k] == P->lrow_slot[r0 + k];
+      P->diag_row_fused[c] = same ? 1 : 0;
+    }
+
   // Components of the symmetrized pattern (union-find)
   {
     std::vector<int> parent(n);
