@@ -118,45 +118,103 @@ __global__ void k_eval(int nm, int B, const double *__restrict__ Girr,
   ieq[gid] = f - g * V;
 }
 
+// Stamping gathers.  A block covers kStampRows consecutive rows (slots of A,
+// or rows of b) for up to 256 instances.  When every row of the block has at
+// most two contributing modules (all rows of a PV plant), the block loads all
+// its module values first and then combines them row by row, so a block is
+// one memory round trip; otherwise the rows are combined one at a time.  The
+// operation order per row is fixed (contributions in map order).
+constexpr int kStampRows = 8;
+
+// out[row] = base[row] + sum_k sign_k * val[mod_k] (+ sign_k * gv per term
+// when gv > 0; SUB: minus instead of plus, without gv).
+template <bool SUB>
+__device__ __forceinline__ void stamp_rows(int rows, int B,
+                                           const double *__restrict__ base,
+                                           const int *__restrict__ ptr,
+                                           const int *__restrict__ mod,
+                                           const int *__restrict__ sgn,
+                                           const double *__restrict__ val,
+                                           double gv, double *__restrict__ out,
+                                           int inst) {
+  const int r0 = (int)blockIdx.x * kStampRows;
+  int kp[kStampRows + 1];
+#pragma unroll
+  for (int r = 0; r <= kStampRows; ++r)
+    kp[r] = ptr[min(r0 + r, rows)];
+  bool simple = true;
+#pragma unroll
+  for (int r = 0; r < kStampRows; ++r)
+    simple = simple && kp[r + 1] - kp[r] <= 2;
+  auto term = [&](double acc, int k, double v) {
+    const double sg = (double)sgn[k];
+    if constexpr (SUB)
+      return acc - sg * v;
+    acc = acc + sg * v;
+    if (gv > 0)
+      acc = acc + sg * gv;
+    return acc;
+  };
+  if (simple) {
+    double v0[kStampRows], v1[kStampRows], bs[kStampRows];
+#pragma unroll
+    for (int r = 0; r < kStampRows; ++r) {
+      const int row = min(r0 + r, rows - 1);
+      bs[r] = base[row];
+      // a row with fewer terms loads module 0 in their place (always valid)
+      const int cnt = kp[r + 1] - kp[r];
+      const int m0 = cnt > 0 ? mod[kp[r]] : 0;
+      const int m1 = cnt > 1 ? mod[kp[r] + 1] : m0;
+      v0[r] = val[(size_t)m0 * B + inst];
+      v1[r] = val[(size_t)m1 * B + inst];
+    }
+#pragma unroll
+    for (int r = 0; r < kStampRows; ++r) {
+      if (r0 + r >= rows)
+        break;
+      const int cnt = kp[r + 1] - kp[r];
+      double acc = bs[r];
+      if (cnt > 0)
+        acc = term(acc, kp[r], v0[r]);
+      if (cnt > 1)
+        acc = term(acc, kp[r] + 1, v1[r]);
+      out[(size_t)(r0 + r) * B + inst] = acc;
+    }
+    return;
+  }
+  for (int r = 0; r < kStampRows && r0 + r < rows; ++r) {
+    double acc = base[r0 + r];
+    for (int k = kp[r]; k < kp[r + 1]; ++k)
+      acc = term(acc, k, val[(size_t)mod[k] * B + inst]);
+    out[(size_t)(r0 + r) * B + inst] = acc;
+  }
+}
+
 // A[q] = linear part + sum over the modules stamping slot q, each adding
 // sign*G and, on homotopy stages, sign*g_virtual.
-__global__ void k_stamp_a(int nnz, int B, const double *__restrict__ base,
-                          const int *__restrict__ ptr,
-                          const int *__restrict__ mod,
-                          const int *__restrict__ sgn,
-                          const double *__restrict__ geq,
-                          const double *__restrict__ gvp, double *A,
-                          const int *active) {
-  PVGLU_ROW_INDEX(B);
-  const int q = row_;
-  if (active && !active[inst])
+__global__ void __launch_bounds__(256)
+    k_stamp_a(int nnz, int B, const double *__restrict__ base,
+              const int *__restrict__ ptr, const int *__restrict__ mod,
+              const int *__restrict__ sgn, const double *__restrict__ geq,
+              const double *__restrict__ gvp, double *__restrict__ A,
+              const int *active) {
+  const int inst = (int)(blockIdx.y * blockDim.x + threadIdx.x);
+  if (inst >= B || (active && !active[inst]))
     return;
-  const double gv = gvp[inst]; // homotopy conductance of this instance's stage
-  double a = base[q];
-  for (int k = ptr[q]; k < ptr[q + 1]; ++k) {
-    const double s = (double)sgn[k];
-    a = a + s * geq[(size_t)mod[k] * B + inst];
-    if (gv > 0)
-      a = a + s * gv;
-  }
-  A[gid] = a;
+  // gvp: homotopy conductance of this instance's stage
+  stamp_rows<false>(nnz, B, base, ptr, mod, sgn, geq, gvp[inst], A, inst);
 }
 
 // b[r] = linear sources - sum over modules of sign * I_eq
-__global__ void k_stamp_b(int n, int B, const double *__restrict__ base,
-                          const int *__restrict__ ptr,
-                          const int *__restrict__ mod,
-                          const int *__restrict__ sgn,
-                          const double *__restrict__ ieq, double *b,
-                          const int *active) {
-  PVGLU_ROW_INDEX(B);
-  const int r = row_;
-  if (active && !active[inst])
+__global__ void __launch_bounds__(256)
+    k_stamp_b(int n, int B, const double *__restrict__ base,
+              const int *__restrict__ ptr, const int *__restrict__ mod,
+              const int *__restrict__ sgn, const double *__restrict__ ieq,
+              double *__restrict__ b, const int *active) {
+  const int inst = (int)(blockIdx.y * blockDim.x + threadIdx.x);
+  if (inst >= B || (active && !active[inst]))
     return;
-  double v = base[r];
-  for (int k = ptr[r]; k < ptr[r + 1]; ++k)
-    v = v - (double)sgn[k] * ieq[(size_t)mod[k] * B + inst];
-  b[gid] = v;
+  stamp_rows<true>(n, B, base, ptr, mod, sgn, ieq, 0.0, b, inst);
 }
 
 // dP/dx on the bus unknowns (permuted): P = sum_bus V G_N (V - 600);
@@ -456,7 +514,9 @@ inline dim3 grid_for(long count) { return dim3((unsigned)((count + 255) / 256));
 int launch_stamp(pvglu_pv *pv, int B, const int *active) {
   NewtonDev *d = pv->nd;
   const int nm = pv->nmod, n = pv->n, nnz = pv->plan->nnz;
-  const RowGrid ge = row_grid(nm, B), ga = row_grid(nnz, B), gb = row_grid(n, B);
+  const RowGrid ge = row_grid(nm, B),
+                ga = row_grid((nnz + kStampRows - 1) / kStampRows, B),
+                gb = row_grid((n + kStampRows - 1) / kStampRows, B);
   hipLaunchKernelGGL(k_eval, ge.grid, ge.block, 0, d->s, nm, B, d->G, d->rsh,
                      d->v, d->geq, d->ieq, active);
   hipLaunchKernelGGL(k_stamp_a, ga.grid, ga.block, 0, d->s, nnz, B, d->base_a,
@@ -836,7 +896,9 @@ extern "C" int pvglu_pv_stamp(pvglu_pv *pv, int batch, const double *g,
   }
   hipLaunchKernelGGL(k_fill_double, grid_for(B), dim3(256), 0, s, d->gvs, B,
                      g_virtual);
-  const RowGrid ge = row_grid(nm, B), ga = row_grid(nnz, B), gb = row_grid(n, B);
+  const RowGrid ge = row_grid(nm, B),
+                ga = row_grid((nnz + kStampRows - 1) / kStampRows, B),
+                gb = row_grid((n + kStampRows - 1) / kStampRows, B);
   hipLaunchKernelGGL(k_eval, ge.grid, ge.block, 0, s, nm, B, g, d->rsh, v, geq,
                      ieq, nullptr);
   hipLaunchKernelGGL(k_stamp_a, ga.grid, ga.block, 0, s, nnz, B, d->base_a,
