@@ -51,7 +51,7 @@ def parse():
                     help="skip the mixed-size (10/40/100 inverters) and 400-inverter legs")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 1 GiB device copy (counter calibration)")
-    ap.add_argument("--schedule", default="level:64:1",
+    ap.add_argument("--schedule", default="hybrid:64:1",
                     help="variant:width:groups of the batched launch (level, component, "
                          "hybrid), or 'auto' for the autotuner's choice")
     ap.add_argument("--cpu-threads", type=int, default=16,

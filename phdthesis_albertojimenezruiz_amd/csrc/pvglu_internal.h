@@ -73,7 +73,7 @@ struct pvglu_plan {
   bool fuse_lsolve = false;
   // per column: the diagonal's updates and row c of the forward sweep share
   // their L operands (one pass computes both sums)
-  std::vector<int> diag_row_fused;
+  std::vector<int> col_flags; // bit 0 fused diagonal/forward row, bit 1 star column
   int ncomp = 0;
   std::vector<int> comp_of; // component id per column
   pvglu::KernelLevels k_lu, k_l, k_u, k_lt;

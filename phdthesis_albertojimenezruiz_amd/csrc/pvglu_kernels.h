@@ -1,0 +1,1083 @@
+// pvglu_kernels.h -- gfx950 kernels of the numeric path (device code).
+//
+// Included only by the per-width translation units pvglu_k<W>.hip, which
+// instantiate Launch<W>; pvglu_device.hip holds the host side.
+//
+// Numeric path (the Newton-step refactor + solve of the reference,
+// PV/factorize_electrical_system_CUDA.cu:61-152, PV/factorize_matrix_GLU_CUDA.cuh:51-211,
+// PV/triangular_substitution.cu:61-271), written in gather form:
+//   * a lane owns one (column, instance) and computes each entry of the column
+//     as  a(i,c) - sum_j L(i,j) U(j,c)  with j ascending, reading operands
+//     that earlier levels (or earlier columns of its chain) finished, so no
+//     atomics are needed and every entry sees the exact operation order of the
+//     sequential CPU GLU (bit-identical results);
+//   * values are slot-major / instance-minor: with `width` lanes of one chain
+//     on consecutive instances, a wavefront reads 64 x 8 B = 512 B contiguous;
+//     when width == 64 all pattern metadata is wavefront-uniform and lives in
+//     scalar registers (readfirstlane), so pattern reads are s_load traffic;
+//   * LEVEL schedule: one launch per level (the reference's launch structure);
+//     COMPONENT schedule: one launch; each workgroup owns one independent
+//     component x instance tile and walks all levels with workgroup barriers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pvglu_launch.h"
+
+#include <cmath>
+#include <type_traits>
+
+namespace pvglu {
+
+template <int W> __device__ __forceinline__ int uniform(int v) {
+  if constexpr (W == 64)
+    return __builtin_amdgcn_readfirstlane(v);
+  else
+    return v;
+}
+
+
+// One column of the refactor for one instance (gather form), optionally
+// followed by the forward substitution of row c.
+// Operation order per entry = GLU_CSC_CPU_sequential's (PV/GLU_CSC_CPU_levels_strings.cpp:32-62):
+// v - L*U for ascending j, then the division by the pivot for i > c.
+// Operand loads are issued in chunks of kChunk independent loads, then the
+// chunk's updates are applied in order: a wide column (the DC bus: 24 updates
+// on the diagonal, a 24-term forward row) costs a few memory latencies instead
+// of one per update.
+constexpr int kChunk = 8;
+
+// One chunk: CH operand pairs loaded unconditionally (indices past the end
+// repeat the last pair), then the updates applied in order.  GET(k) returns
+// the two operands of update k.
+template <int CH, class GET>
+__device__ __forceinline__ double chunk_sub(double v, int kb, int k1, GET get) {
+  double l[CH], r[CH];
+#pragma unroll
+  for (int j = 0; j < CH; ++j)
+    get(min(kb + j, k1 - 1), l[j], r[j]);
+#pragma unroll
+  for (int j = 0; j < CH; ++j)
+    if (kb + j < k1)
+      v = v - l[j] * r[j];
+  return v;
+}
+
+// v - sum_k l_k r_k over [k0, k1) in ascending k: full chunks of kChunk
+// independent loads, then one chunk sized to the remainder, so a sum costs
+// ceil(n / kChunk) memory round trips and issues at most 2 n + kChunk loads.
+template <int CH = kChunk, class GET>
+__device__ __forceinline__ double dot_sub(double v, int k0, int k1, GET get) {
+  int kb = k0;
+  for (; k1 - kb >= CH; kb += CH)
+    v = chunk_sub<CH>(v, kb, k1, get);
+  const int rest = k1 - kb;
+  if (rest > CH / 2)
+    v = chunk_sub<CH>(v, kb, k1, get);
+  else if (rest > 2)
+    v = chunk_sub<(CH / 2 > 2 ? CH / 2 : 2)>(v, kb, k1, get);
+  else if (rest == 2)
+    v = chunk_sub<2>(v, kb, k1, get);
+  else if (rest == 1)
+    v = chunk_sub<1>(v, kb, k1, get);
+  return v;
+}
+
+// Two sums sharing their first operand: v - sum_k l_k r_k and
+// y - sum_k l_k s_k, each in ascending k (the same operation order as two
+// separate dot_sub calls), from one set of loads.
+template <int CH, class GET>
+__device__ __forceinline__ void chunk_sub2(double &v, double &y, int kb, int k1,
+                                           GET get) {
+  double l[CH], r[CH], t[CH];
+#pragma unroll
+  for (int j = 0; j < CH; ++j)
+    get(min(kb + j, k1 - 1), l[j], r[j], t[j]);
+#pragma unroll
+  for (int j = 0; j < CH; ++j)
+    if (kb + j < k1) {
+      v = v - l[j] * r[j];
+      y = y - l[j] * t[j];
+    }
+}
+
+template <class GET>
+__device__ __forceinline__ void dot_sub2(double &v, double &y, int k0, int k1,
+                                         GET get) {
+  int kb = k0;
+  for (; k1 - kb >= kChunk; kb += kChunk)
+    chunk_sub2<kChunk>(v, y, kb, k1, get);
+  const int rest = k1 - kb;
+  if (rest > kChunk / 2)
+    chunk_sub2<kChunk>(v, y, kb, k1, get);
+  else if (rest > 2)
+    chunk_sub2<kChunk / 2>(v, y, kb, k1, get);
+  else if (rest > 0)
+    chunk_sub2<2>(v, y, kb, k1, get);
+}
+
+template <int W>
+__device__ __forceinline__ double dot_sub_chunked(double v, const int *sl_arr,
+                                                  const int *sr_arr,
+                                                  const double *L,
+                                                  const double *R, int k0,
+                                                  int k1, size_t B, int inst) {
+  return dot_sub(v, k0, k1, [&](int k, double &l, double &r) {
+    l = L[(size_t)uniform<W>(sl_arr[k]) * B + inst];
+    r = R[(size_t)uniform<W>(sr_arr[k]) * B + inst];
+  });
+}
+
+// Same over consecutive CSC slots q of LU against y at the rows ri[q].
+template <int W>
+__device__ __forceinline__ double dot_sub_csc(double v, const int *ri,
+                                              const double *LU,
+                                              const double *y, int q0, int q1,
+                                              size_t B, int inst) {
+  return dot_sub(v, q0, q1, [&](int q, double &l, double &r) {
+    l = LU[(size_t)q * B + inst];
+    r = y[(size_t)uniform<W>(ri[q]) * B + inst];
+  });
+}
+
+template <int W, bool FUSE>
+__device__ __forceinline__ void refactor_column(const Pat &P, int c,
+                                                const double *A, double *LU,
+                                                const double *b, double *x,
+                                                size_t B, int inst) {
+  const int q0 = uniform<W>(P.cp[c]);
+  const int q1 = uniform<W>(P.cp[c + 1]);
+  const int qd = uniform<W>(P.diag[c]);
+  double piv = 1.0;
+  bool row_with_diag = false; // forward row computed with the diagonal
+  double yrow = 0.0;
+  if constexpr (FUSE)
+    row_with_diag = (uniform<W>(P.drf[c]) & 1) != 0;
+  // the A values of CH entries are loaded together, then the entries are
+  // updated in slot order
+  auto entries = [&](auto ch, int qb) {
+    constexpr int CH = decltype(ch)::value;
+    const int cnt = min(CH, q1 - qb);
+    double a[CH];
+    int optr[CH + 1]; // the chunk's update ranges, loaded together
+#pragma unroll
+    for (int k = 0; k < CH; ++k)
+      a[k] = A[(size_t)min(qb + k, q1 - 1) * B + inst];
+#pragma unroll
+    for (int k = 0; k <= CH; ++k)
+      optr[k] = uniform<W>(P.op_ptr[min(qb + k, q1)]);
+#pragma unroll
+    for (int k = 0; k < CH; ++k) {
+      if (k >= cnt)
+        break;
+      const int q = qb + k;
+      double v = a[k];
+      const int o0 = optr[k], o1 = optr[k + 1];
+      if (FUSE && row_with_diag && q == qd) {
+        const int r0 = uniform<W>(P.lrow_ptr[c]);
+        yrow = b[(size_t)c * B + inst];
+        dot_sub2(v, yrow, o0, o1, [&](int o, double &l, double &u, double &yj) {
+          l = LU[(size_t)uniform<W>(P.op_l[o]) * B + inst];
+          u = LU[(size_t)uniform<W>(P.op_u[o]) * B + inst];
+          yj = x[(size_t)uniform<W>(P.lrow_col[r0 + (o - o0)]) * B + inst];
+        });
+      } else {
+        v = dot_sub_chunked<W>(v, P.op_l, P.op_u, LU, LU, o0, o1, B, inst);
+      }
+      if (q == qd) {
+        piv = v;
+        if (fabs(v) < 1e-8)
+          record_status(P.status, PVGLU_E_SINGULAR, c, inst);
+      } else if (q > qd) {
+        v = v / piv;
+      }
+      LU[(size_t)q * B + inst] = v;
+    }
+  };
+  for (int qb = q0; qb < q1; qb += kChunk) {
+    if (q1 - qb > kChunk / 2)
+      entries(std::integral_constant<int, kChunk>(), qb);
+    else
+      entries(std::integral_constant<int, kChunk / 2>(), qb);
+  }
+  if constexpr (FUSE) {
+    if (row_with_diag)
+      x[(size_t)c * B + inst] = yrow;
+    else
+      x[(size_t)c * B + inst] = dot_sub_chunked<W>(
+          b[(size_t)c * B + inst], P.lrow_slot, P.lrow_col, LU, x,
+          uniform<W>(P.lrow_ptr[c]), uniform<W>(P.lrow_ptr[c + 1]), B, inst);
+  }
+}
+
+// Star column (column flag bit 1; the DC bus): only the diagonal has
+// updates, so the entries above the diagonal are their A values, every U
+// operand of the diagonal is such an A value, and the entries below the
+// diagonal are A / pivot; the operands' slots and rows are arithmetic
+// progressions, addressed from their first two values (no per-operand index
+// loads).  All operands of the diagonal's sum (and of the fused forward row)
+// are loaded together, before anything of the column is stored, and the sums
+// run branch-free over a fixed register budget of kStarMax updates: the
+// column costs one memory round trip instead of one per operand chunk.  Used
+// by the narrow-run kernel, where the column is on a latency-bound chain.
+// Operation order per entry = refactor_column's.
+constexpr int kStarMax = 32;
+
+template <int W, bool FUSE>
+__device__ __forceinline__ void star_column(const Pat &P, int c,
+                                            const double *A, double *LU,
+                                            const double *b, double *x,
+                                            size_t B, int inst) {
+  const int q0 = uniform<W>(P.cp[c]);
+  const int q1 = uniform<W>(P.cp[c + 1]);
+  const int qd = uniform<W>(P.diag[c]);
+  const int flags = uniform<W>(P.drf[c]);
+  const bool row_with_diag = FUSE && (flags & 1) != 0;
+  const int r0 = FUSE ? uniform<W>(P.lrow_ptr[c]) : 0;
+  const int od0 = uniform<W>(P.op_ptr[qd]);
+  const int nd = uniform<W>(P.op_ptr[qd + 1]) - od0; // 1 .. kStarMax
+  const int o1 = od0 + min(1, nd - 1);
+  const int l0 = uniform<W>(P.op_l[od0]), dl = uniform<W>(P.op_l[o1]) - l0;
+  const int u0 = uniform<W>(P.op_u[od0]), du = uniform<W>(P.op_u[o1]) - u0;
+  int y0 = 0, dy = 0;
+  if (row_with_diag) {
+    y0 = uniform<W>(P.lrow_col[r0]);
+    dy = uniform<W>(P.lrow_col[r0 + (o1 - od0)]) - y0;
+  }
+  double l[kStarMax], u[kStarMax], yv[kStarMax];
+#pragma unroll
+  for (int j = 0; j < kStarMax; ++j) {
+    const int k = min(j, nd - 1);
+    l[j] = LU[(size_t)(l0 + k * dl) * B + inst];
+    u[j] = A[(size_t)(u0 + k * du) * B + inst];
+    if (row_with_diag)
+      yv[j] = x[(size_t)(y0 + k * dy) * B + inst];
+  }
+  double v = A[(size_t)qd * B + inst];
+  double y = row_with_diag ? b[(size_t)c * B + inst] : 0.0;
+  // entries above the diagonal: no updates (LU = A), copied in chunks
+  for (int qb = q0; qb < qd; qb += 8) {
+    double t[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      t[k] = A[(size_t)min(qb + k, qd - 1) * B + inst];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (qb + k < qd)
+        LU[(size_t)(qb + k) * B + inst] = t[k];
+  }
+#pragma unroll
+  for (int j = 0; j < kStarMax; ++j) {
+    const double pv = v - l[j] * u[j];
+    v = j < nd ? pv : v;
+    if (row_with_diag) {
+      const double py = y - l[j] * yv[j];
+      y = j < nd ? py : y;
+    }
+  }
+  if (fabs(v) < 1e-8)
+    record_status(P.status, PVGLU_E_SINGULAR, c, inst);
+  LU[(size_t)qd * B + inst] = v;
+  // entries below the diagonal: no updates, divided by the pivot
+  for (int qb = qd + 1; qb < q1; qb += 4) {
+    double t[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      t[k] = A[(size_t)min(qb + k, q1 - 1) * B + inst];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (qb + k < q1)
+        LU[(size_t)(qb + k) * B + inst] = t[k] / v;
+  }
+  if constexpr (FUSE) {
+    if (row_with_diag)
+      x[(size_t)c * B + inst] = y;
+    else
+      x[(size_t)c * B + inst] = dot_sub_chunked<W>(
+          b[(size_t)c * B + inst], P.lrow_slot, P.lrow_col, LU, x, r0,
+          uniform<W>(P.lrow_ptr[c + 1]), B, inst);
+  }
+}
+
+// Row i of the forward substitution: y_i = b_i - sum_j L(i,j) y_j (j asc),
+// the per-target order of Process_Column_L_CSC_CPU sweeps (PV/GLU_CSC_CPU_Triangular.cpp:3-11,27-33).
+template <int W>
+__device__ __forceinline__ void lsolve_row(const Pat &P, int i,
+                                           const double *LU, const double *b,
+                                           double *x, size_t B, int inst) {
+  x[(size_t)i * B + inst] = dot_sub_chunked<W>(
+      b[(size_t)i * B + inst], P.lrow_slot, P.lrow_col, LU, x,
+      uniform<W>(P.lrow_ptr[i]), uniform<W>(P.lrow_ptr[i + 1]), B, inst);
+}
+
+// Row i of the backward substitution: x_i = (y_i - sum_j U(i,j) x_j)/U(i,i)
+// with j descending, the per-target order of Process_Column_U_CSC_CPU sweeps
+// (PV/GLU_CSC_CPU_Triangular.cpp:13-25,35-39).
+template <int W>
+__device__ __forceinline__ void usolve_row(const Pat &P, int i,
+                                           const double *LU, double *x,
+                                           size_t B, int inst) {
+  const int d = uniform<W>(P.diag[i]);
+  const double dv = LU[(size_t)d * B + inst];
+  const double s = dot_sub_chunked<W>(
+      x[(size_t)i * B + inst], P.urow_slot, P.urow_col, LU, x,
+      uniform<W>(P.urow_ptr[i]), uniform<W>(P.urow_ptr[i + 1]), B, inst);
+  x[(size_t)i * B + inst] = s / dv;
+}
+
+// Adjoint, U^T z = g: z_j = (g_j - sum_{i<j} U(i,j) z_i) / U(j,j), the CSC
+// column j above its diagonal (contiguous slots), i ascending.
+template <int W>
+__device__ __forceinline__ void utsolve_col(const Pat &P, int j,
+                                            const double *LU, const double *g,
+                                            double *y, size_t B, int inst) {
+  const int q0 = uniform<W>(P.cp[j]);
+  const int d = uniform<W>(P.diag[j]);
+  const double s =
+      dot_sub_csc<W>(g[(size_t)j * B + inst], P.ri, LU, y, q0, d, B, inst);
+  y[(size_t)j * B + inst] = s / LU[(size_t)d * B + inst];
+}
+
+// Adjoint, L^T y = z: y_j = z_j - sum_{i>j} L(i,j) y_i, column j below its
+// diagonal, i ascending.
+template <int W>
+__device__ __forceinline__ void ltsolve_col(const Pat &P, int j,
+                                            const double *LU, double *y,
+                                            size_t B, int inst) {
+  const int d = uniform<W>(P.diag[j]);
+  const int q1 = uniform<W>(P.cp[j + 1]);
+  y[(size_t)j * B + inst] =
+      dot_sub_csc<W>(y[(size_t)j * B + inst], P.ri, LU, y, d + 1, q1, B, inst);
+}
+
+
+// ------------------------------------------------ register-forwarding chains
+// Runs of columns with <= 4 entries execute from the 16-int records built by
+// compile_fast_programs: the column's values live in registers, operands that
+// the previous column of the run produced are forwarded from registers, and
+// the A and b values two columns ahead are loaded while the current column
+// computes.  A strided segment shares one record: its columns' slots and rows
+// are computed, so inside a segment no pattern data is loaded at all.
+// Each entry still sees the generic path's operation sequence (bit-identical).
+
+__device__ __forceinline__ double sel4(double v0, double v1, double v2,
+                                       double v3, int i) {
+  return i == 0 ? v0 : (i == 1 ? v1 : (i == 2 ? v2 : v3));
+}
+
+template <int W, int N>
+__device__ __forceinline__ void load_record(const int *r, int (&f)[N]) {
+  const int4 *r4 = reinterpret_cast<const int4 *>(r);
+#pragma unroll
+  for (int k = 0; k < N / 4; ++k) {
+    const int4 v = r4[k];
+    f[4 * k] = uniform<W>(v.x);
+    f[4 * k + 1] = uniform<W>(v.y);
+    f[4 * k + 2] = uniform<W>(v.z);
+    f[4 * k + 3] = uniform<W>(v.w);
+  }
+}
+
+// A and b values of one refactor column.
+struct ColVals {
+  double a0, a1, a2, a3, bb;
+};
+
+template <bool FUSE>
+__device__ __forceinline__ void load_col(ColVals &c, int slot0, int col, int ne,
+                                         const double *A, const double *b,
+                                         size_t B, int inst) {
+  c.a0 = A[(size_t)slot0 * B + inst];
+  c.a1 = ne > 1 ? A[(size_t)(slot0 + 1) * B + inst] : 0.0;
+  c.a2 = ne > 2 ? A[(size_t)(slot0 + 2) * B + inst] : 0.0;
+  c.a3 = ne > 3 ? A[(size_t)(slot0 + 3) * B + inst] : 0.0;
+  c.bb = FUSE ? b[(size_t)col * B + inst] : 0.0;
+}
+
+// A segment whose operands are all forwarded from the previous column runs
+// as a three-stage pipeline with a fixed sequence of memory operations per
+// column (the loads of column i+2 are issued before column i computes, stores
+// are unconditional in NE, the loop is unrolled three times so no register
+// copies of in-flight loads are needed): the compiler's counter-based waits
+// then wait for the column being computed only, not for the prefetches.
+template <int NE, bool FUSE>
+__device__ __forceinline__ void pure_load_col(ColVals &c, int slot0, int col,
+                                              const double *A,
+                                              const double *b, size_t B,
+                                              int inst) {
+  c.a0 = A[(size_t)slot0 * B + inst];
+  if constexpr (NE > 1)
+    c.a1 = A[(size_t)(slot0 + 1) * B + inst];
+  if constexpr (NE > 2)
+    c.a2 = A[(size_t)(slot0 + 2) * B + inst];
+  if constexpr (NE > 3)
+    c.a3 = A[(size_t)(slot0 + 3) * B + inst];
+  if constexpr (FUSE)
+    c.bb = b[(size_t)col * B + inst];
+}
+
+template <int NE, bool FUSE>
+__device__ __forceinline__ void pure_lu_col(const ColVals &c, int slot0,
+                                            int col, const int (&f)[16],
+                                            double *LU, double *x, size_t B,
+                                            int inst, double &p0, double &p1,
+                                            double &p2, double &p3,
+                                            double &yprev, int &badcol) {
+  const int meta = f[2];
+  const int d = (meta >> 3) & 7, nops = (meta >> 6) & 7, nlr = (meta >> 9) & 3;
+  double v0 = c.a0, v1 = NE > 1 ? c.a1 : 0.0, v2 = NE > 2 ? c.a2 : 0.0,
+         v3 = NE > 3 ? c.a3 : 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (k < nops) {
+      const int tu = f[5 + 2 * k];
+      const double l = sel4(p0, p1, p2, p3, -1 - f[4 + 2 * k]);
+      const double prod = l * sel4(v0, v1, v2, v3, tu >> 2);
+      switch (tu & 3) {
+      case 0: v0 = v0 - prod; break;
+      case 1: v1 = v1 - prod; break;
+      case 2: v2 = v2 - prod; break;
+      default: v3 = v3 - prod; break;
+      }
+    }
+  }
+  const double piv = sel4(v0, v1, v2, v3, d);
+  if (fabs(piv) < 1e-8 && badcol < 0)
+    badcol = col;
+  if (NE > 1 && d < 1)
+    v1 = v1 / piv;
+  if (NE > 2 && d < 2)
+    v2 = v2 / piv;
+  if (NE > 3 && d < 3)
+    v3 = v3 / piv;
+  LU[(size_t)slot0 * B + inst] = v0;
+  if constexpr (NE > 1)
+    LU[(size_t)(slot0 + 1) * B + inst] = v1;
+  if constexpr (NE > 2)
+    LU[(size_t)(slot0 + 2) * B + inst] = v2;
+  if constexpr (NE > 3)
+    LU[(size_t)(slot0 + 3) * B + inst] = v3;
+  if constexpr (FUSE) {
+    double y = c.bb;
+    if (nlr > 0)
+      y = y - sel4(p0, p1, p2, p3, -1 - f[12]) * yprev;
+    x[(size_t)col * B + inst] = y;
+    yprev = y;
+  }
+  p0 = v0;
+  p1 = v1;
+  p2 = v2;
+  p3 = v3;
+}
+
+// Software pipeline over the steps i of a segment: the loads of step i + D
+// are issued before step i computes, into a ring of D + 1 register buffers
+// indexed by unrolled constants (no copies of registers with loads in
+// flight); LD clamps its step to the segment.
+template <int D, class T, class LD, class RUN>
+__device__ __forceinline__ void pipeline(int len, LD ld, RUN run) {
+  T buf[D + 1];
+#pragma unroll
+  for (int k = 0; k < D; ++k)
+    ld(buf[k], k);
+  for (int i = 0;;) {
+#pragma unroll
+    for (int u = 0; u <= D; ++u) {
+      ld(buf[(u + D) % (D + 1)], i + D);
+      run(buf[u], i);
+      if (++i >= len)
+        return;
+    }
+  }
+}
+
+// Prefetch depth of the refactor and substitution register pipelines: 2 in
+// launches whose occupancy matters (wide levels), kNarrow*Depth in the
+// narrow-run kernel, where few wavefronts run and each step of a chain would
+// otherwise wait for a memory round trip issued only two steps earlier.
+constexpr int kLuDepth = 2, kRowDepth = 2;
+constexpr int kNarrowLuDepth = 8, kNarrowRowDepth = 8;
+
+template <int NE, bool FUSE, int D = kLuDepth>
+__device__ __forceinline__ void pure_lu_segment(
+    const int (&f)[16], int len, int dslot, int dcol, const double *A,
+    double *LU, const double *b, double *x, size_t B, int inst, double &p0,
+    double &p1, double &p2, double &p3, double &yprev, int &badcol) {
+  pipeline<D, ColVals>(
+      len,
+      [&](ColVals &c, int i) {
+        i = min(i, len - 1);
+        pure_load_col<NE, FUSE>(c, f[0] + i * dslot, f[1] + i * dcol, A, b, B,
+                                inst);
+      },
+      [&](const ColVals &c, int i) {
+        pure_lu_col<NE, FUSE>(c, f[0] + i * dslot, f[1] + i * dcol, f, LU, x,
+                              B, inst, p0, p1, p2, p3, yprev, badcol);
+      });
+}
+
+// Tridiagonal step of a series string (segment flags bits 2/3): entries
+// (U above, diagonal, L below); the diagonal minus the previous column's
+// entry k times the U entry is the pivot, the L entry is divided by it, and
+// the fused forward row subtracts the same L operand times the previous y.
+// The same operations as pure_lu_col on such a record, without its
+// run-time record decoding (selects and branches per update).
+template <bool FUSE, int D>
+__device__ __forceinline__ void tri_lu_segment(
+    int slot0, int col0, int len, int dslot, int dcol, int k, const double *A,
+    double *LU, const double *b, double *x, size_t B, int inst, double &p0,
+    double &p1, double &p2, double &p3, double &yprev, int &badcol) {
+  double pl = sel4(p0, p1, p2, p3, k);
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+  pipeline<D, ColVals>(
+      len,
+      [&](ColVals &c, int i) {
+        i = min(i, len - 1);
+        pure_load_col<3, FUSE>(c, slot0 + i * dslot, col0 + i * dcol, A, b, B,
+                               inst);
+      },
+      [&](const ColVals &c, int i) {
+        const size_t s = (size_t)(slot0 + i * dslot) * B + inst;
+        const double u = c.a0;
+        const double piv = c.a1 - pl * u;
+        if (fabs(piv) < 1e-8 && badcol < 0)
+          badcol = col0 + i * dcol;
+        const double lo = c.a2 / piv;
+        LU[s] = u;
+        LU[s + B] = piv;
+        LU[s + 2 * B] = lo;
+        if constexpr (FUSE) {
+          const double y = c.bb - pl * yprev;
+          x[(size_t)(col0 + i * dcol) * B + inst] = y;
+          yprev = y;
+        }
+        v0 = u;
+        v1 = piv;
+        v2 = lo;
+        pl = k == 0 ? u : (k == 1 ? piv : (k == 2 ? lo : 0.0));
+      });
+  p0 = v0;
+  p1 = v1;
+  p2 = v2;
+  p3 = 0.0;
+}
+
+// Backward-substitution values of one row: y, U(i,i) and the two U terms.
+struct RowVals {
+  double y, dv, u0, u1;
+};
+
+// Row sweeps share one register program: x_i = (src_i - sum_j T(i,j) x_j)
+// [/ T(i,i) when DIV], T = U (backward, src = x in place, DIV) or unit L
+// (forward, src = b).  Records hold slots relative to the row's diagonal slot.
+template <int NT, bool DIV, int D = kRowDepth>
+__device__ __forceinline__ void pure_row_segment(const int (&f)[8], int len,
+                                                 int drow, int ddiag,
+                                                 const double *LU,
+                                                 const double *src, double *x,
+                                                 size_t B, int inst,
+                                                 double &xprev) {
+  pipeline<D, RowVals>(
+      len,
+      [&](RowVals &r, int i) {
+        i = min(i, len - 1);
+        const int row = f[0] + i * drow, dslot = f[1] + i * ddiag;
+        r.y = src[(size_t)row * B + inst];
+        if constexpr (DIV)
+          r.dv = LU[(size_t)dslot * B + inst];
+        if constexpr (NT > 0)
+          r.u0 = LU[(size_t)(dslot + f[4]) * B + inst];
+      },
+      [&](const RowVals &r, int i) {
+        double s = r.y;
+        if constexpr (NT > 0)
+          s = s - r.u0 * xprev;
+        const double xi = DIV ? s / r.dv : s;
+        x[(size_t)(f[0] + i * drow) * B + inst] = xi;
+        xprev = xi;
+      });
+}
+
+template <int W, bool FUSE, int D = kLuDepth>
+__device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
+                                              const double *A, double *LU,
+                                              const double *b, double *x,
+                                              size_t B, int inst) {
+  double p0 = 0, p1 = 0, p2 = 0, p3 = 0, yprev = 0;
+  for (int t = t0; t < t1;) {
+    const int *sg = P.lu_seg + (size_t)t * 4;
+    const int len = uniform<W>(sg[0]);
+    const int dslot = uniform<W>(sg[1]);
+    const int dcol = uniform<W>(sg[2]);
+    int f[16];
+    load_record<W>(P.lu_rec + (size_t)t * 16, f);
+    const int meta = f[2];
+    const int ne = meta & 7, d = (meta >> 3) & 7, nops = (meta >> 6) & 7,
+              nlr = (meta >> 9) & 3;
+    const int sflag = uniform<W>(sg[3]);
+    if (sflag & (FUSE ? 8 : 4)) {
+      int badcol = -1;
+      tri_lu_segment<FUSE, D>(f[0], f[1], len, dslot, dcol, (sflag >> 4) & 3,
+                              A, LU, b, x, B, inst, p0, p1, p2, p3, yprev,
+                              badcol);
+      if (badcol >= 0)
+        record_status(P.status, PVGLU_E_SINGULAR, badcol, inst);
+      t += len;
+      continue;
+    }
+    if (sflag & (FUSE ? 1 : 2)) {
+      int badcol = -1;
+      switch (ne) {
+      case 1:
+        pure_lu_segment<1, FUSE, D>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
+                           p2, p3, yprev, badcol);
+        break;
+      case 2:
+        pure_lu_segment<2, FUSE, D>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
+                           p2, p3, yprev, badcol);
+        break;
+      case 3:
+        pure_lu_segment<3, FUSE, D>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
+                           p2, p3, yprev, badcol);
+        break;
+      default:
+        pure_lu_segment<4, FUSE, D>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
+                           p2, p3, yprev, badcol);
+        break;
+      }
+      if (badcol >= 0)
+        record_status(P.status, PVGLU_E_SINGULAR, badcol, inst);
+      t += len;
+      continue;
+    }
+    ColVals c0, c1, c2;
+    load_col<FUSE>(c0, f[0], f[1], ne, A, b, B, inst);
+    if (len > 1)
+      load_col<FUSE>(c1, f[0] + dslot, f[1] + dcol, ne, A, b, B, inst);
+    for (int i = 0; i < len; ++i) {
+      if (i + 2 < len)
+        load_col<FUSE>(c2, f[0] + (i + 2) * dslot, f[1] + (i + 2) * dcol, ne,
+                       A, b, B, inst);
+      const int slot0 = f[0] + i * dslot, col = f[1] + i * dcol;
+      double v0 = c0.a0, v1 = c0.a1, v2 = c0.a2, v3 = c0.a3;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (k < nops) {
+          const int lsrc = f[4 + 2 * k], tu = f[5 + 2 * k];
+          const double l = lsrc >= 0 ? LU[(size_t)lsrc * B + inst]
+                                     : sel4(p0, p1, p2, p3, -1 - lsrc);
+          const double prod = l * sel4(v0, v1, v2, v3, tu >> 2);
+          switch (tu & 3) {
+          case 0: v0 = v0 - prod; break;
+          case 1: v1 = v1 - prod; break;
+          case 2: v2 = v2 - prod; break;
+          default: v3 = v3 - prod; break;
+          }
+        }
+      }
+      const double piv = sel4(v0, v1, v2, v3, d);
+      if (fabs(piv) < 1e-8)
+        record_status(P.status, PVGLU_E_SINGULAR, col, inst);
+      if (d < 1 && ne > 1)
+        v1 = v1 / piv;
+      if (d < 2 && ne > 2)
+        v2 = v2 / piv;
+      if (d < 3 && ne > 3)
+        v3 = v3 / piv;
+      LU[(size_t)slot0 * B + inst] = v0;
+      if (ne > 1)
+        LU[(size_t)(slot0 + 1) * B + inst] = v1;
+      if (ne > 2)
+        LU[(size_t)(slot0 + 2) * B + inst] = v2;
+      if (ne > 3)
+        LU[(size_t)(slot0 + 3) * B + inst] = v3;
+      if constexpr (FUSE) { // forward substitution of row col
+        double y = c0.bb;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+          if (k < nlr) {
+            const int lsrc = f[12 + 2 * k], ysrc = f[13 + 2 * k];
+            const double l = lsrc >= 0 ? LU[(size_t)lsrc * B + inst]
+                                       : sel4(p0, p1, p2, p3, -1 - lsrc);
+            const double yj = ysrc >= 0 ? x[(size_t)ysrc * B + inst] : yprev;
+            y = y - l * yj;
+          }
+        }
+        x[(size_t)col * B + inst] = y;
+        yprev = y;
+      }
+      p0 = v0;
+      p1 = v1;
+      p2 = v2;
+      p3 = v3;
+      c0 = c1;
+      c1 = c2;
+    }
+    t += len;
+  }
+}
+
+template <bool DIV>
+__device__ __forceinline__ void load_row(RowVals &v, int row, int dslot, int nt,
+                                         int rel0, int rel1, const double *LU,
+                                         const double *src, size_t B,
+                                         int inst) {
+  v.y = src[(size_t)row * B + inst];
+  v.dv = DIV ? LU[(size_t)dslot * B + inst] : 1.0;
+  v.u0 = nt > 0 ? LU[(size_t)(dslot + rel0) * B + inst] : 0.0;
+  v.u1 = nt > 1 ? LU[(size_t)(dslot + rel1) * B + inst] : 0.0;
+}
+
+template <int W, bool DIV, int D = kRowDepth>
+__device__ __forceinline__ void fast_row_chain(const int *rec, const int *seg,
+                                               int t0, int t1,
+                                               const double *LU,
+                                               const double *src, double *x,
+                                               size_t B, int inst) {
+  double xprev = 0;
+  for (int t = t0; t < t1;) {
+    const int *sg = seg + (size_t)t * 4;
+    const int len = uniform<W>(sg[0]);
+    const int drow = uniform<W>(sg[1]);
+    const int ddiag = uniform<W>(sg[2]);
+    int f[8];
+    load_record<W>(rec + (size_t)t * 8, f);
+    const int nt = f[2];
+    if (uniform<W>(sg[3])) {
+      if (nt == 0)
+        pure_row_segment<0, DIV, D>(f, len, drow, ddiag, LU, src, x, B, inst,
+                                 xprev);
+      else
+        pure_row_segment<1, DIV, D>(f, len, drow, ddiag, LU, src, x, B, inst,
+                                 xprev);
+      t += len;
+      continue;
+    }
+    RowVals r0, r1, r2;
+    load_row<DIV>(r0, f[0], f[1], nt, f[4], f[6], LU, src, B, inst);
+    if (len > 1)
+      load_row<DIV>(r1, f[0] + drow, f[1] + ddiag, nt, f[4], f[6], LU, src, B,
+                    inst);
+    for (int i = 0; i < len; ++i) {
+      if (i + 2 < len)
+        load_row<DIV>(r2, f[0] + (i + 2) * drow, f[1] + (i + 2) * ddiag, nt,
+                      f[4], f[6], LU, src, B, inst);
+      const int row = f[0] + i * drow;
+      double s = r0.y;
+      if (nt > 0)
+        s = s - r0.u0 * (f[5] >= 0 ? x[(size_t)f[5] * B + inst] : xprev);
+      if (nt > 1)
+        s = s - r0.u1 * (f[7] >= 0 ? x[(size_t)f[7] * B + inst] : xprev);
+      const double xi = DIV ? s / r0.dv : s;
+      x[(size_t)row * B + inst] = xi;
+      xprev = xi;
+      r0 = r1;
+      r1 = r2;
+    }
+    t += len;
+  }
+}
+
+
+// NAR = 1: the narrow-run kernel's variant (deep register pipelines, star
+// columns loaded in one round trip).
+template <int W, int OP, int NAR = 0>
+__device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
+                                          const double *A, double *LU,
+                                          const double *b, double *x, size_t B,
+                                          int inst) {
+  constexpr int DL = NAR ? kNarrowLuDepth : kLuDepth;
+  constexpr int DR = NAR ? kNarrowRowDepth : kRowDepth;
+  const int t0 = uniform<W>(L.chain_col_ptr[chain]);
+  const int t1 = uniform<W>(L.chain_col_ptr[chain + 1]);
+  if constexpr (OP == OP_REFACTOR_FUSED || OP == OP_REFACTOR) {
+    constexpr bool FUSE = OP == OP_REFACTOR_FUSED;
+    if (uniform<W>(P.lu_fast[chain])) {
+      fast_lu_chain<W, FUSE, DL>(P, t0, t1, A, LU, b, x, B, inst);
+      return;
+    }
+    // mixed chain: register runs between generic columns
+    for (int t = t0; t < t1;) {
+      const int e = uniform<W>(P.lu_run[t]);
+      if (e > t) {
+        fast_lu_chain<W, FUSE, DL>(P, t, e, A, LU, b, x, B, inst);
+        t = e;
+      } else {
+        const int c = uniform<W>(L.cols[t]);
+        bool star = false;
+        if constexpr (NAR != 0)
+          star = (uniform<W>(P.drf[c]) & 2) != 0;
+        if constexpr (NAR != 0) {
+          if (star)
+            star_column<W, FUSE>(P, c, A, LU, b, x, B, inst);
+        }
+        if (!star)
+          refactor_column<W, FUSE>(P, c, A, LU, b, x, B, inst);
+        ++t;
+      }
+    }
+    return;
+  }
+  if constexpr (OP == OP_USOLVE || OP == OP_LSOLVE || OP == OP_UT ||
+                OP == OP_LT) {
+    // row programs: backward (U), forward (L), adjoint U^T (divides) and L^T
+    constexpr bool DIV = OP == OP_USOLVE || OP == OP_UT;
+    const int *rec, *seg, *run, *whole;
+    if constexpr (OP == OP_USOLVE) {
+      rec = P.u_rec, seg = P.u_seg, run = P.u_run, whole = P.u_fast;
+    } else if constexpr (OP == OP_LSOLVE) {
+      rec = P.l_rec, seg = P.l_seg, run = P.l_run, whole = P.l_fast;
+    } else if constexpr (OP == OP_UT) {
+      rec = P.ut_rec, seg = P.ut_seg, run = P.ut_run, whole = P.ut_fast;
+    } else {
+      rec = P.lt_rec, seg = P.lt_seg, run = P.lt_run, whole = P.lt_fast;
+    }
+    // source of the right-hand side: forward b, U^T g; in place otherwise
+    const double *src = (OP == OP_LSOLVE || OP == OP_UT) ? b : x;
+    if (uniform<W>(whole[chain])) {
+      fast_row_chain<W, DIV, DR>(rec, seg, t0, t1, LU, src, x, B, inst);
+      return;
+    }
+    for (int t = t0; t < t1;) {
+      const int e = uniform<W>(run[t]);
+      if (e > t) {
+        fast_row_chain<W, DIV, DR>(rec, seg, t, e, LU, src, x, B, inst);
+        t = e;
+      } else {
+        const int c = uniform<W>(L.cols[t]);
+        if constexpr (OP == OP_USOLVE)
+          usolve_row<W>(P, c, LU, x, B, inst);
+        else if constexpr (OP == OP_LSOLVE)
+          lsolve_row<W>(P, c, LU, b, x, B, inst);
+        else if constexpr (OP == OP_UT)
+          utsolve_col<W>(P, c, LU, b, x, B, inst);
+        else
+          ltsolve_col<W>(P, c, LU, x, B, inst);
+        ++t;
+      }
+    }
+    return;
+  }
+  for (int t = t0; t < t1; ++t) {
+    const int c = uniform<W>(L.cols[t]);
+    if constexpr (OP == OP_REFACTOR)
+      refactor_column<W, false>(P, c, A, LU, b, x, B, inst);
+    else if constexpr (OP == OP_REFACTOR_FUSED)
+      refactor_column<W, true>(P, c, A, LU, b, x, B, inst);
+    else if constexpr (OP == OP_LSOLVE)
+      lsolve_row<W>(P, c, LU, b, x, B, inst);
+    else if constexpr (OP == OP_USOLVE)
+      usolve_row<W>(P, c, LU, x, B, inst);
+    else if constexpr (OP == OP_UT)
+      utsolve_col<W>(P, c, LU, b, x, B, inst);
+    else
+      ltsolve_col<W>(P, c, LU, x, B, inst);
+  }
+}
+
+// LEVEL schedule: one launch per level.  Thread -> (chain group, lane);
+// group -> (instance tile, chain of the level).  The backward sweeps (U and
+// L^T) take the groups in descending order: a wide backward level re-reads
+// the factors and forward results that the wide forward level wrote shortly
+// before, and in reverse order its first blocks read what was written last,
+// which the 256 MB Infinity Cache still holds (measured: 10 us less per
+// 512-instance refactor+solve of the 40x24x28 plant).
+template <int W, int OP>
+__global__ void __launch_bounds__(256)
+    k_level(Pat P, Lev L, int first_chain, int nchains, int tiles,
+            const double *A, double *LU, const double *b, double *x, int B) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g0 = uniform<W>(gid / W);
+  const int lane = gid % W;
+  if (g0 >= nchains * tiles)
+    return;
+  constexpr bool backward = OP == OP_USOLVE || OP == OP_LT;
+  const int group = backward ? nchains * tiles - 1 - g0 : g0;
+  // consecutive wavefronts take consecutive instance tiles of one chain, so
+  // the tiles of a slot row are streamed together
+  const int tile = group % tiles;
+  const int chain = first_chain + group / tiles;
+  const int inst = tile * W + lane;
+  if (inst >= B)
+    return;
+  run_chain<W, OP>(P, L, chain, A, LU, b, x, (size_t)B, inst);
+}
+
+// COMPONENT schedule: one launch for the whole refactor + solve.  Block ->
+// (component, instance tile); each level of the component is split over the
+// block's chain slots; barriers separate levels.  Every thread reaches every
+// barrier (loop bounds are block-uniform; inactive lanes only skip work).
+template <int W, int NAR>
+__device__ __forceinline__ void component_levels(const Pat &P, const Lev &L,
+                                                 int comp, int slot, int slots,
+                                                 bool active, const double *A,
+                                                 double *LU, const double *b,
+                                                 double *x, size_t B, int inst,
+                                                 int op, int lv_lo = 0,
+                                                 int lv_hi = 1 << 30) {
+  const int e0 = L.comp_ptr[comp], e1 = L.comp_ptr[comp + 1];
+  for (int e = e0; e < e1; ++e) {
+    const int lv = L.comp_lv_level[e];
+    if (lv < lv_lo || lv >= lv_hi)
+      continue; // block-uniform: no work, no barrier
+    const int c0 = L.comp_lv_begin[e], c1 = L.comp_lv_end[e];
+    if (active)
+      for (int ch = c0 + slot; ch < c1; ch += slots) {
+        const int chain = uniform<W>(ch);
+        if (op == OP_REFACTOR_FUSED)
+          run_chain<W, OP_REFACTOR_FUSED, NAR>(P, L, chain, A, LU, b, x, B, inst);
+        else if (op == OP_REFACTOR)
+          run_chain<W, OP_REFACTOR, NAR>(P, L, chain, A, LU, b, x, B, inst);
+        else if (op == OP_LSOLVE)
+          run_chain<W, OP_LSOLVE, NAR>(P, L, chain, A, LU, b, x, B, inst);
+        else if (op == OP_USOLVE)
+          run_chain<W, OP_USOLVE, NAR>(P, L, chain, A, LU, b, x, B, inst);
+        else if (op == OP_UT)
+          run_chain<W, OP_UT, NAR>(P, L, chain, A, LU, b, x, B, inst);
+        else
+          run_chain<W, OP_LT, NAR>(P, L, chain, A, LU, b, x, B, inst);
+      }
+    __syncthreads();
+  }
+}
+
+// mode 0: refactor only; 1: refactor + solve; 2: solve only; 3: transposed solve
+template <int W>
+__device__ __forceinline__ void component_body(const Pat &P, const Lev &Llu,
+                                               const Lev &Ll, const Lev &Lu,
+                                               const Lev &Llt, int mode,
+                                               bool fuse, int comp, int tile,
+                                               const double *A, double *LU,
+                                               const double *b, double *x,
+                                               int B) {
+  const int slots = blockDim.x / W;
+  const int slot = uniform<W>((int)threadIdx.x / W);
+  const int lane = threadIdx.x % W;
+  const int inst = tile * W + lane;
+  const bool active = inst < B;
+  const size_t Bs = (size_t)B;
+  if (mode == 3) {
+    component_levels<W, 0>(P, Llu, comp, slot, slots, active, A, LU, b, x,
+                               Bs, inst, OP_UT);
+    component_levels<W, 0>(P, Llt, comp, slot, slots, active, A, LU, b, x,
+                               Bs, inst, OP_LT);
+    return;
+  }
+  if (mode <= 1)
+    component_levels<W, 0>(P, Llu, comp, slot, slots, active, A, LU, b, x,
+                               Bs, inst,
+                               (mode == 1 && fuse) ? OP_REFACTOR_FUSED
+                                                   : OP_REFACTOR);
+  if (mode == 0)
+    return;
+  if (mode == 2 || !fuse)
+    component_levels<W, 0>(P, Ll, comp, slot, slots, active, A, LU, b, x,
+                               Bs, inst, OP_LSOLVE);
+  component_levels<W, 0>(P, Lu, comp, slot, slots, active, A, LU, b, x, Bs,
+                             inst, OP_USOLVE);
+}
+
+template <int W>
+__global__ void __launch_bounds__(1024)
+    k_component(Pat P, Lev Llu, Lev Ll, Lev Lu, Lev Llt, int mode, bool fuse,
+                int tiles, const double *A, double *LU, const double *b,
+                double *x, int B) {
+  component_body<W>(P, Llu, Ll, Lu, Llt, mode, fuse, blockIdx.x / tiles,
+                    blockIdx.x % tiles, A, LU, b, x, B);
+}
+
+// HYBRID schedule: a run of narrow levels (few chains) in one launch.  Block
+// -> (component, tile); the four level sets are walked in order
+// (refactor, forward, backward, transposed-L) over the given level ranges,
+// each with its operation, barriers between levels.  Few wavefronts run, so
+// the chains use the deep register pipelines and star columns take the
+// one-round-trip path.  At most 4 chain slots of W <= 64 lanes (a narrow
+// level has at most 4 chains per component), so at most 256 threads.
+template <int W>
+__global__ void __launch_bounds__(256)
+    k_hybrid(Pat P, Lev Llu, Lev Ll, Lev Lu, Lev Llt, LvRange R, int comp0,
+             int tiles, const double *A, double *LU, const double *b,
+             double *x, int B) {
+  const int comp = comp0 + blockIdx.x / tiles, tile = blockIdx.x % tiles;
+  const int slots = blockDim.x / W;
+  const int slot = uniform<W>((int)threadIdx.x / W);
+  const int inst = tile * W + (int)threadIdx.x % W;
+  const bool active = inst < B;
+  // level set k selected field by field (no indexed array of kernel
+  // arguments, which would live in scratch memory)
+  for (int k = 0; k < 4; ++k) {
+    const int lo = k == 0 ? R.lo[0] : k == 1 ? R.lo[1] : k == 2 ? R.lo[2] : R.lo[3];
+    const int hi = k == 0 ? R.hi[0] : k == 1 ? R.hi[1] : k == 2 ? R.hi[2] : R.hi[3];
+    const int op = k == 0 ? R.op[0] : k == 1 ? R.op[1] : k == 2 ? R.op[2] : R.op[3];
+    if (lo >= hi)
+      continue;
+    const Lev Lk = k == 0 ? Llu : k == 1 ? Ll : k == 2 ? Lu : Llt;
+    component_levels<W, 1>(P, Lk, comp, slot, slots, active, A, LU, b, x,
+                           (size_t)B, inst, op, lo, hi);
+  }
+}
+
+// Multi-pattern work list: groups of instances with their own plans in one
+// launch; block -> (group, component, instance tile).
+
+template <int W>
+__global__ void __launch_bounds__(1024) k_multi(MultiArgs m, int mode) {
+  int g = 0;
+  while (g + 1 < m.ngroups && (int)blockIdx.x >= m.block_start[g + 1])
+    ++g;
+  const int local = blockIdx.x - m.block_start[g];
+  component_body<W>(m.pat[g], m.lu[g], m.l[g], m.u[g], m.lt[g], mode,
+                    m.fuse[g] != 0, local / m.tiles[g], local % m.tiles[g],
+                    m.a[g], m.luv[g], m.b[g], m.x[g], m.B[g]);
+}
+
+template <int W>
+void Launch<W>::level(const Pat &pat, const Lev &lev, int op, int first,
+                      int nch, int tiles, const double *A, double *LU,
+                      const double *b, double *x, int B, hipStream_t s) {
+  const long threads = (long)nch * tiles * W;
+  const int blocks = (int)((threads + 255) / 256);
+  if (blocks == 0)
+    return;
+#define PVGLU_LVL(OPC)                                                         \
+  hipLaunchKernelGGL((k_level<W, OPC>), dim3(blocks), dim3(256), 0, s, pat,    \
+                     lev, first, nch, tiles, A, LU, b, x, B)
+  switch (op) {
+  case OP_REFACTOR: PVGLU_LVL(OP_REFACTOR); break;
+  case OP_REFACTOR_FUSED: PVGLU_LVL(OP_REFACTOR_FUSED); break;
+  case OP_LSOLVE: PVGLU_LVL(OP_LSOLVE); break;
+  case OP_USOLVE: PVGLU_LVL(OP_USOLVE); break;
+  case OP_UT: PVGLU_LVL(OP_UT); break;
+  default: PVGLU_LVL(OP_LT); break;
+  }
+#undef PVGLU_LVL
+}
+
+template <int W>
+void Launch<W>::hybrid(const Pat &pat, const Lev &lu, const Lev &l,
+                       const Lev &u, const Lev &lt, const LvRange &R,
+                       int comp0, int ncomp, int tiles, int threads,
+                       const double *A, double *LU, const double *b, double *x,
+                       int B, hipStream_t s) {
+  if (ncomp > 0)
+    hipLaunchKernelGGL((k_hybrid<W>), dim3(ncomp * tiles), dim3(threads), 0, s,
+                       pat, lu, l, u, lt, R, comp0, tiles, A, LU, b, x, B);
+}
+
+template <int W>
+void Launch<W>::component(const Pat &pat, const Lev &lu, const Lev &l,
+                          const Lev &u, const Lev &lt, int mode, bool fuse,
+                          int ncomp, int tiles, int threads, const double *A,
+                          double *LU, const double *b, double *x, int B,
+                          hipStream_t s) {
+  hipLaunchKernelGGL((k_component<W>), dim3(ncomp * tiles), dim3(threads), 0,
+                     s, pat, lu, l, u, lt, mode, fuse, tiles, A, LU, b, x, B);
+}
+
+template <int W>
+void Launch<W>::multi(const MultiArgs &m, int mode, int blocks, int threads,
+                      hipStream_t s) {
+  hipLaunchKernelGGL((k_multi<W>), dim3(blocks), dim3(threads), 0, s, m, mode);
+}
+
+} // namespace pvglu

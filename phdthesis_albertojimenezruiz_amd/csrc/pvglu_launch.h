@@ -1,0 +1,124 @@
+// pvglu_launch.h -- kernel argument views of a device plan and the
+// per-width launchers of the numeric kernels.
+//
+// The refactor / substitution kernels are templates over the instance width
+// W (lanes of one chain on consecutive instances).  Each width is compiled in
+// its own translation unit (pvglu_k<W>.hip instantiates Launch<W>), so the
+// widths build in parallel; host code calls Launch<W> through these
+// declarations only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "pvglu.h"
+
+namespace pvglu {
+
+struct Pat {
+  const int *__restrict__ cp;
+  const int *__restrict__ ri;
+  const int *__restrict__ op_ptr;
+  const int *__restrict__ op_l;
+  const int *__restrict__ op_u;
+  const int *__restrict__ diag;
+  const int *__restrict__ lrow_ptr;
+  const int *__restrict__ lrow_slot;
+  const int *__restrict__ lrow_col;
+  const int *__restrict__ urow_ptr;
+  const int *__restrict__ urow_slot;
+  const int *__restrict__ urow_col;
+  const int *__restrict__ lu_rec;
+  const int *__restrict__ lu_seg;
+  const int *__restrict__ lu_run;
+  const int *__restrict__ lu_fast;
+  const int *__restrict__ u_rec;
+  const int *__restrict__ u_seg;
+  const int *__restrict__ u_run;
+  const int *__restrict__ u_fast;
+  const int *__restrict__ l_rec;
+  const int *__restrict__ l_seg;
+  const int *__restrict__ l_run;
+  const int *__restrict__ l_fast;
+  const int *__restrict__ ut_rec;
+  const int *__restrict__ ut_seg;
+  const int *__restrict__ ut_run;
+  const int *__restrict__ ut_fast;
+  const int *__restrict__ lt_rec;
+  const int *__restrict__ lt_seg;
+  const int *__restrict__ lt_run;
+  const int *__restrict__ lt_fast;
+  const int *__restrict__ drf;
+  int *status;
+};
+
+struct Lev {
+  const int *__restrict__ level_chain_ptr;
+  const int *__restrict__ chain_col_ptr;
+  const int *__restrict__ cols;
+  const int *__restrict__ comp_ptr;
+  const int *__restrict__ comp_lv_begin;
+  const int *__restrict__ comp_lv_end;
+  const int *__restrict__ comp_lv_level;
+};
+
+enum Op { OP_REFACTOR = 0, OP_REFACTOR_FUSED, OP_LSOLVE, OP_USOLVE, OP_UT, OP_LT };
+
+// HYBRID schedule: a run of narrow levels (few chains) in one launch.  Block
+// -> (component, tile); the four level sets are walked in order
+// (refactor, forward, backward, transposed-L) over the given level ranges,
+// each with its operation, barriers between levels.
+struct LvRange {
+  int lo[4], hi[4], op[4];
+};
+
+// Multi-pattern work list: groups of instances with their own plans in one
+// launch; block -> (group, component, instance tile).
+constexpr int kMaxGroups = 6;
+struct MultiArgs {
+  int ngroups;
+  int block_start[kMaxGroups + 1];
+  int tiles[kMaxGroups];
+  int B[kMaxGroups];
+  int fuse[kMaxGroups];
+  Pat pat[kMaxGroups];
+  Lev lu[kMaxGroups], l[kMaxGroups], u[kMaxGroups], lt[kMaxGroups];
+  const double *a[kMaxGroups];
+  double *luv[kMaxGroups];
+  const double *b[kMaxGroups];
+  double *x[kMaxGroups];
+};
+
+__device__ __forceinline__ void record_status(int *status, int code, int col,
+                                              int inst) {
+  if (atomicCAS(status, 0, code) == 0) {
+    status[1] = col;
+    status[2] = inst;
+  }
+}
+
+// Launchers of the numeric kernels at instance width W.  All are stream
+// ordered and neither synchronize nor allocate.
+template <int W> struct Launch {
+  // one level (chains [first, first + nch) of a level set) with operation op;
+  // grid: ceil(nch * tiles * W / 256) blocks of 256 threads
+  static void level(const Pat &pat, const Lev &lev, int op, int first,
+                    int nch, int tiles, const double *A, double *LU,
+                    const double *b, double *x, int B, hipStream_t s);
+  // a run of narrow levels, block -> (component comp0 + k, tile)
+  static void hybrid(const Pat &pat, const Lev &lu, const Lev &l,
+                     const Lev &u, const Lev &lt, const LvRange &R, int comp0,
+                     int ncomp, int tiles, int threads, const double *A,
+                     double *LU, const double *b, double *x, int B,
+                     hipStream_t s);
+  // the whole refactor / solve, block -> (component, tile)
+  static void component(const Pat &pat, const Lev &lu, const Lev &l,
+                        const Lev &u, const Lev &lt, int mode, bool fuse,
+                        int ncomp, int tiles, int threads, const double *A,
+                        double *LU, const double *b, double *x, int B,
+                        hipStream_t s);
+  // multi-pattern work list
+  static void multi(const MultiArgs &m, int mode, int blocks, int threads,
+                    hipStream_t s);
+};
+
+} // namespace pvglu

@@ -253,6 +253,9 @@ void build_kernel_levels(const LevelSet &ls, const std::vector<int> &comp_of,
 // run[t] = end position of the register run that contains t (0 if t does not
 // qualify); chain flag = 1 when one run covers the whole chain.
 static constexpr int kFastMaxE = 4, kFastMaxOps = 4, kFastMaxLr = 2;
+// star columns: at most this many updates on the diagonal (the kernel's
+// register budget for the star path)
+static constexpr int kStarMaxOps = 32;
 
 // With the fused forward substitution the record also carries the L row of
 // the column (at most kFastMaxLr terms); without it the row is left out.
@@ -426,15 +429,27 @@ void compile_fast_programs(pvglu_plan *P) {
   // operands.  Such a segment issues no pattern-addressed loads: the kernel
   // runs it as a static three-stage load pipeline (bit 0 with the fused
   // forward substitution, bit 1 for the refactor alone).
+  // Bits 2 and 3 mark the tridiagonal step of a series string (the common
+  // record): three entries (U above, diagonal, L below), one update of the
+  // diagonal by the previous column's entry k (bits 4-5) times the U entry,
+  // and (bit 3) one forward term with the same L operand times the previous
+  // y.  The kernel runs such segments from a branch-free specialization.
   compile_segments(P->fast_lu_rec, 16, P->fast_lu_run, P->n, P->fast_lu_seg,
                    [](const int *r) {
+                     const int ne = r[2] & 7, d = (r[2] >> 3) & 7;
                      const int nops = (r[2] >> 6) & 7, nlr = (r[2] >> 9) & 3;
                      for (int k = 0; k < nops; k++)
                        if (r[4 + 2 * k] >= 0)
                          return 0;
                      const bool lrow_fwd =
                          nlr == 0 || (nlr == 1 && r[12] < 0 && r[13] == -1);
-                     return lrow_fwd ? 3 : 2;
+                     int flags = lrow_fwd ? 3 : 2;
+                     if (ne == 3 && d == 1 && nops == 1 && r[5] == 1) {
+                       flags |= 4 | ((-1 - r[4]) << 4);
+                       if (nlr == 1 && r[12] == r[4] && r[13] == -1)
+                         flags |= 8;
+                     }
+                     return flags;
                    });
   auto row_forwarded = [](const int *r) {
     return (r[2] == 0 || (r[2] == 1 && r[5] == -1)) ? 1 : 0;
@@ -656,19 +671,42 @@ extern "C" int pvglu_plan_create(int n, const int *rowptr, const int *colidx,
     }
     P->fuse_lsolve = ok;
   }
-  // Columns whose diagonal updates and forward-substitution row use the same
-  // L operands in the same order compute both sums in one pass.
-  P->diag_row_fused.assign(n, 0);
-  if (P->fuse_lsolve)
-    for (int c = 0; c < n; c++) {
-      const int qd = P->diag_slot[c];
-      const int o0 = P->op_ptr[qd], o1 = P->op_ptr[qd + 1];
+  // Column flags.  Bit 0: the diagonal updates and the forward-substitution
+  // row use the same L operands in the same order (both sums in one pass).
+  // Bit 1: a star column -- only the diagonal has updates (1 to kStarMaxOps
+  // of them), so every U operand is an entry above the diagonal without
+  // updates, i.e. its A value, and the update operands are arithmetic
+  // progressions: L slots, U slots and (fused row) forward rows each with a
+  // constant step (the DC bus column: one update per string eliminated before
+  // it, the strings' combiner columns numbered consecutively).  The kernel
+  // addresses them from the first two of each.
+  P->col_flags.assign(n, 0);
+  for (int c = 0; c < n; c++) {
+    const int qd = P->diag_slot[c];
+    const int o0 = P->op_ptr[qd], o1 = P->op_ptr[qd + 1];
+    if (P->fuse_lsolve) {
       const int r0 = P->lrow_ptr[c], r1 = P->lrow_ptr[c + 1];
       bool same = o1 - o0 == r1 - r0;
       for (int k = 0; same && k < o1 - o0; k++)
         same = P->op_l[o0 + k] == P->lrow_slot[r0 + k];
-      P->diag_row_fused[c] = same ? 1 : 0;
+      P->col_flags[c] |= same ? 1 : 0;
     }
+    const int q0 = P->cp[c], q1 = P->cp[c + 1];
+    bool star = o1 - o0 >= 1 && o1 - o0 <= kStarMaxOps &&
+                P->op_ptr[q1] - P->op_ptr[q0] == o1 - o0;
+    auto progression = [&](const int *v, int len) {
+      for (int k = 2; k < len; k++)
+        if (v[k] - v[k - 1] != v[1] - v[0])
+          return false;
+      return true;
+    };
+    if (star)
+      star = progression(&P->op_l[o0], o1 - o0) &&
+             progression(&P->op_u[o0], o1 - o0) &&
+             (!(P->col_flags[c] & 1) ||
+              progression(&P->lrow_col[P->lrow_ptr[c]], o1 - o0));
+    P->col_flags[c] |= star ? 2 : 0;
+  }
 
   // Components of the symmetrized pattern (union-find)
   {
