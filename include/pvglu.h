@@ -128,6 +128,16 @@ enum {
   PVGLU_ARR_U_START_CHAINS,
   PVGLU_ARR_U_CHAINS,
   PVGLU_ARR_MAX_PER_LEVEL,      /* max_elements_per_column             nlev_lu */
+  /* kernel programs of this build (no counterpart in the reference): */
+  PVGLU_ARR_COL_FLAGS,          /* per column: bit 0 diagonal and forward row
+                                   share their L operands, bit 1 star column  n */
+  PVGLU_ARR_LU_SEGMENTS,        /* refactor register program, per position in
+                                   kernel order: (length, slot step, column
+                                   step, flags) at a segment start, else 0;
+                                   flags: bit 0/1 operands forwarded (with /
+                                   without the forward row), bit 2/3 the
+                                   tridiagonal step (bits 4-5: the previous
+                                   column's forwarded entry)          4n */
   PVGLU_ARR_COUNT
 };
 /* Length of an exported array, or a negative status. */
@@ -169,10 +179,11 @@ int pvglu_get_schedule(const pvglu_plan *plan, int *variant, int *width);
 /* Component groups of the LEVEL and HYBRID schedules (1, 2 or 4; default 1).
  * With G > 1 the independent components are split into G groups whose
  * launches run on the plan's own side streams, forked from and joined back
- * into the caller's stream with events (capturable), each group starting
- * after the previous group's first level.  The side streams and events
- * belong to the plan: two numeric calls on one plan from different host
- * threads at the same time need G = 1.  No counterpart in the reference. */
+ * into the caller's stream with events, each group starting after the
+ * previous group's first level.  On a stream under capture the setting is
+ * ignored (one group).  The side streams and events belong to the plan: two
+ * numeric calls on one plan from different host threads at the same time
+ * need G = 1.  No counterpart in the reference. */
 int pvglu_set_stream_groups(pvglu_plan *plan, int groups);
 int pvglu_get_stream_groups(const pvglu_plan *plan);
 

@@ -31,7 +31,8 @@ INFO = dict(n=0, nnz=1, nlev_lu=2, nchain_lu=3, nlev_l=4, nchain_l=5, nlev_u=6,
 ARRAYS = dict(csr_rowptr=0, csr_colidx=1, csc_colptr=2, csc_rowidx=3, csr2csc=4,
               csc2csr=5, csr_diag=6, csc_diag=7, lu_start_levels=8, lu_start_chains=9,
               lu_chains=10, l_start_levels=11, l_start_chains=12, l_chains=13,
-              u_start_levels=14, u_start_chains=15, u_chains=16, max_per_level=17)
+              u_start_levels=14, u_start_chains=15, u_chains=16, max_per_level=17,
+              col_flags=18, lu_segments=19)
 
 _P_INT = C.POINTER(C.c_int)
 _P_DBL = C.POINTER(C.c_double)

@@ -418,6 +418,15 @@ static int launch_steps_w(pvglu_plan *P, int mode, bool hybrid, int G,
   DevicePlan *d = P->dev;
   if (G > 1 && !d->side_ok)
     G = 1;
+  if (G > 1) {
+    // under stream capture the groups are not forked: they gain nothing on
+    // MI355X (eager or captured, measured) and a 4-stream fork/join during
+    // capture has crashed the host runtime
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_TRY(hipStreamIsCapturing(s, &cs));
+    if (cs != hipStreamCaptureStatusNone)
+      G = 1;
+  }
   const int gi = group_index(G);
   G = 1 << gi;
   const DevLevels *sets[4] = {&d->lu, &d->l, &d->u, &d->lt};

@@ -7,7 +7,7 @@
 // and exchanged as a fixture.
 //
 // Layout, all integers little-endian int32 unless stated:
-//   "PVGLUPLN" (8 bytes), version (=1), n, nnz, section count
+//   "PVGLUPLN" (8 bytes), version (=2), n, nnz, section count
 //   per section: id, length L, then L int32 values
 //     ids 0..PVGLU_ARR_COUNT-1: the pvglu_plan_export arrays
 //     id 100: P (length n or 0), id 101: Prow (length n or 0)
@@ -26,7 +26,7 @@ using namespace pvglu;
 namespace {
 
 constexpr char kMagic[8] = {'P', 'V', 'G', 'L', 'U', 'P', 'L', 'N'};
-constexpr int kVersion = 1;
+constexpr int kVersion = 2;
 constexpr int kSecP = 100, kSecProw = 101;
 
 uint64_t fnv1a(uint64_t h, const void *p, size_t n) {

@@ -801,6 +801,8 @@ static const std::vector<int> *plan_array(const pvglu_plan *P, int which) {
   case PVGLU_ARR_U_START_CHAINS: return &P->u.start_chains;
   case PVGLU_ARR_U_CHAINS: return &P->u.chains;
   case PVGLU_ARR_MAX_PER_LEVEL: return &P->max_per_level;
+  case PVGLU_ARR_COL_FLAGS: return &P->col_flags;
+  case PVGLU_ARR_LU_SEGMENTS: return &P->fast_lu_seg;
   default: return nullptr;
   }
 }

@@ -38,6 +38,49 @@ def plant_pattern(I, S, M):
     return n, rows
 
 
+def hub_pattern(lengths, tail):
+    """A hub node (a DC bus) tied to the top node of len(lengths) strings
+    numbered before it (each from its leaf up), then a tail string numbered
+    after it from its top down.  In natural order: zero fill, the hub
+    column's updates all on its diagonal (one per string before it), the
+    strings' columns tridiagonal steps.  Returns (n, row sets)."""
+    rows, tops = [], []
+
+    def node():
+        rows.append(set([len(rows)]))
+        return len(rows) - 1
+
+    def link(a, c):
+        rows[a].add(c)
+        rows[c].add(a)
+
+    for L in lengths:
+        prev = None
+        for _ in range(L):
+            k = node()
+            if prev is not None:
+                link(prev, k)
+            prev = k
+        tops.append(prev)
+    hub = node()
+    for t in tops:
+        link(t, hub)
+    prev = hub
+    for _ in range(tail):
+        k = node()
+        link(prev, k)
+        prev = k
+    return len(rows), rows
+
+
+# (string lengths before the hub, tail length): star columns with 1, 2, 23
+# and 32 updates; 33 updates (over the star register budget) and unequal
+# strings (operands not strided) take the generic gather
+HUB_CASES = [([5], 0), ([5, 5], 7), ([4] * 23, 27), ([3] * 32, 6), ([3] * 33, 6),
+             ([3, 5, 4, 6], 5)]
+HUB_STAR = [True, True, True, True, False, False]
+
+
 def random_pattern(rng, n, density, symmetric=False, diag=True):
     rows = [set() for _ in range(n)]
     m = rng.random((n, n)) < density

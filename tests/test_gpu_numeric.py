@@ -12,8 +12,8 @@ import pytest
 
 import phdthesis_albertojimenezruiz_amd as pv
 from oracle import oracle as O
-from tests._patterns import (csr_from_sets, dd_values, plant_pattern, plant_values,
-                             random_pattern)
+from tests._patterns import (HUB_CASES, csr_from_sets, dd_values, hub_pattern,
+                             plant_pattern, plant_values, random_pattern)
 
 pytestmark = pytest.mark.gpu
 
@@ -222,6 +222,53 @@ def test_plant_refactor_solve(dev, cfg, B, sched):
     plan.solve(dlu2, db, dx2, B)
     torch.cuda.synchronize()
     assert plan.status()[0] == 0
+    assert torch.equal(dlu2, dlu) and torch.equal(dx2, dx)
+
+
+def _hub_case(lengths, tail):
+    n, rows = hub_pattern(lengths, tail)
+    rp, ci = csr_from_sets(rows)
+    frp, fci = O.symbolic_fill(n, rp, ci)
+    assert len(fci) == len(ci)                    # a tree in this order: no fill
+    return n, frp, fci
+
+
+@pytest.mark.parametrize("case", range(len(HUB_CASES)))
+@pytest.mark.parametrize("sched", [(pv.SCHED_HYBRID, 64), (pv.SCHED_HYBRID, 1),
+                                   (pv.SCHED_LEVEL, 64), (pv.SCHED_COMPONENT, 64)])
+@pytest.mark.parametrize("B", [1, 70])
+def test_star_and_tridiagonal_paths_bitexact(dev, case, sched, B):
+    """Star columns with 1, 2, 23, 32 updates (33: over the register budget,
+    generic gather; unequal strings: operands not strided, generic gather),
+    the tridiagonal segments before and after them, fused and unfused
+    refactor, both sweeps and the transposed sweeps, against the oracle."""
+    n, rp, ci = _hub_case(*HUB_CASES[case])
+    rng = np.random.default_rng(100 + case)
+    plan = pv.Plan(n, rp, ci)
+    plan.set_schedule(*sched)
+    s = O.csr_csc(n, rp, ci)
+    A = np.stack([dd_values(rng, n, rp, ci)[s.csc2csr] for _ in range(B)], axis=1)
+    bb = rng.normal(size=(n, B))
+    want_lu, want_x, want_t = np.zeros_like(A), np.zeros_like(bb), np.zeros_like(bb)
+    for k in range(B):
+        want_lu[:, k], want_x[:, k] = _oracle_lu_solve(s, A[:, k].copy(), bb[:, k].copy())
+        g = bb[:, k].copy()
+        O.solve_transpose_csc(s, want_lu[:, k].copy(), g)
+        want_t[:, k] = g
+    dA, db = _to_dev(A, dev), _to_dev(bb, dev)
+    dlu, dx, dy = torch.empty_like(dA), torch.empty_like(db), torch.empty_like(db)
+    plan.status_reset()
+    plan.refactor_solve(dA, dlu, db, dx, B)
+    plan.solve_transpose(dlu, db, dy, B)
+    torch.cuda.synchronize()
+    assert plan.status()[0] == 0
+    assert np.array_equal(dlu.cpu().numpy(), want_lu)
+    assert np.array_equal(dx.cpu().numpy(), want_x)
+    assert np.array_equal(dy.cpu().numpy(), want_t)
+    dlu2, dx2 = torch.empty_like(dA), torch.empty_like(db)
+    plan.refactor(dA, dlu2, B)
+    plan.solve(dlu2, db, dx2, B)
+    torch.cuda.synchronize()
     assert torch.equal(dlu2, dlu) and torch.equal(dx2, dx)
 
 

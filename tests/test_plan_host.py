@@ -5,7 +5,8 @@ import pytest
 
 import phdthesis_albertojimenezruiz_amd as pv
 from oracle import oracle as O
-from tests._patterns import csr_from_sets, plant_pattern, random_pattern
+from tests._patterns import (HUB_CASES, HUB_STAR, csr_from_sets, hub_pattern, plant_pattern,
+                             random_pattern)
 
 
 def _cases():
@@ -88,3 +89,62 @@ def test_numeric_entry_points_fail_without_device():
     with pytest.raises(pv.PvgluError) as e:
         plan.refactor(1, 2, 1)
     assert e.value.code == pv.E_HIP
+
+
+def _plan_of(n, rows, amd=False):
+    rp, ci = csr_from_sets(rows)
+    if amd:
+        P, _ = pv.reorder(n, rp, ci)
+        rp, ci, _ = O.permute_sym(n, rp, ci, np.zeros(len(ci)), P)
+    rp, ci = O.symbolic_fill(n, rp, ci)
+    return pv.Plan(n, rp, ci), rp, ci
+
+
+@pytest.mark.parametrize("case", range(len(HUB_CASES)))
+def test_star_column_classification_hubs(case):
+    """Column flag bit 1 (star column: updates only on the diagonal, at most
+    32, strided operands) is set on the hub exactly when its strings have
+    equal lengths and there are at most 32 of them.  A string column is a
+    one-update star (its pivot update), the bottom column of a string has no
+    update and is not.  Bit 0 (diagonal and forward row share their L
+    operands) holds on every column of these trees."""
+    lengths, tail = HUB_CASES[case]
+    n, rows = hub_pattern(lengths, tail)
+    plan, _, _ = _plan_of(n, rows)
+    flags = plan.array("col_flags")
+    hub = sum(lengths)
+    assert bool(flags[hub] & 2) == HUB_STAR[case]
+    bottoms = set(np.cumsum([0] + lengths[:-1]).tolist())
+    for c in range(n):
+        if c != hub:
+            assert bool(flags[c] & 2) == (c not in bottoms), c
+    assert np.all(flags & 1)
+
+
+def test_star_columns_and_tridiagonal_segments_of_the_plant():
+    """40x24x28 plant in AMD order: the star columns are exactly the 40 DC
+    bus columns (25 entries: 24 combiners and the bus), and the strings'
+    columns run as tridiagonal segments forwarding the previous column's L
+    entry (segment flags bits 2 and 3, entry 2)."""
+    I, S, M = 40, 24, 28
+    n, rows = plant_pattern(I, S, M)
+    plan, rp, ci = _plan_of(n, rows, amd=True)
+    flags = plan.array("col_flags")
+    cp = plan.array("csc_colptr")
+    nnz_col = np.diff(cp)
+    buses = np.nonzero(nnz_col == S + 1)[0]
+    assert len(buses) == I
+    assert np.all(flags[buses] & 2)
+    # no other column with more than the register path's 4 entries
+    assert np.array_equal(np.nonzero(nnz_col > 4)[0], buses)
+    seg = plan.array("lu_segments").reshape(-1, 4)
+    starts = seg[seg[:, 0] > 0]
+    tri = starts[(starts[:, 3] & 12) == 12]
+    # forwarded operand: entry 1 of the bottom column (diagonal, L), entry 2
+    # of a tridiagonal column (U, diagonal, L)
+    assert set(((tri[:, 3] >> 4) & 3).tolist()) == {1, 2}
+    # every string column but the bottom one is a tridiagonal step, except
+    # two per inverter whose forwarded operand is in memory: the column after
+    # the bus (a star column, outside the register path) and the last
+    # string's column after its bottom node (eliminated a level earlier)
+    assert tri[:, 0].sum() == I * S * (M - 1) - 2 * I
