@@ -140,11 +140,16 @@ __device__ __forceinline__ double dot_sub_csc(double v, const int *ri,
   });
 }
 
+// On return p0..p3 hold the column's last four entries (entries ne-4 .. ne-1)
+// and yprev its forward-substitution value: a register run that follows the
+// column takes operands from them (records encoded against the window).
 template <int W, bool FUSE>
 __device__ __forceinline__ void refactor_column(const Pat &P, int c,
                                                 const double *A, double *LU,
                                                 const double *b, double *x,
-                                                size_t B, int inst) {
+                                                size_t B, int inst, double &p0,
+                                                double &p1, double &p2,
+                                                double &p3, double &yprev) {
   const int q0 = uniform<W>(P.cp[c]);
   const int q1 = uniform<W>(P.cp[c + 1]);
   const int qd = uniform<W>(P.diag[c]);
@@ -192,6 +197,10 @@ __device__ __forceinline__ void refactor_column(const Pat &P, int c,
         v = v / piv;
       }
       LU[(size_t)q * B + inst] = v;
+      p0 = p1;
+      p1 = p2;
+      p2 = p3;
+      p3 = v;
     }
   };
   for (int qb = q0; qb < q1; qb += kChunk) {
@@ -201,101 +210,12 @@ __device__ __forceinline__ void refactor_column(const Pat &P, int c,
       entries(std::integral_constant<int, kChunk / 2>(), qb);
   }
   if constexpr (FUSE) {
-    if (row_with_diag)
-      x[(size_t)c * B + inst] = yrow;
-    else
-      x[(size_t)c * B + inst] = dot_sub_chunked<W>(
-          b[(size_t)c * B + inst], P.lrow_slot, P.lrow_col, LU, x,
-          uniform<W>(P.lrow_ptr[c]), uniform<W>(P.lrow_ptr[c + 1]), B, inst);
-  }
-}
-
-// Star column (column flag bit 1; the DC bus): only the diagonal has
-// updates, so the entries above the diagonal are their A values, every U
-// operand of the diagonal is such an A value, and the entries below the
-// diagonal are A / pivot; the operands' slots and rows are arithmetic
-// progressions, addressed from their first two values (no per-operand index
-// loads).  All operands of the diagonal's sum (and of the fused forward row)
-// are loaded together, before anything of the column is stored, and the sums
-// run branch-free over a fixed register budget of kStarMax updates: the
-// column costs one memory round trip instead of one per operand chunk.  Used
-// by the narrow-run kernel, where the column is on a latency-bound chain.
-// Operation order per entry = refactor_column's.
-constexpr int kStarMax = 32;
-
-template <int W, bool FUSE>
-__device__ __forceinline__ void star_column(const Pat &P, int c,
-                                            const double *A, double *LU,
-                                            const double *b, double *x,
-                                            size_t B, int inst) {
-  const int q0 = uniform<W>(P.cp[c]);
-  const int q1 = uniform<W>(P.cp[c + 1]);
-  const int qd = uniform<W>(P.diag[c]);
-  const int flags = uniform<W>(P.drf[c]);
-  const bool row_with_diag = FUSE && (flags & 1) != 0;
-  const int r0 = FUSE ? uniform<W>(P.lrow_ptr[c]) : 0;
-  const int od0 = uniform<W>(P.op_ptr[qd]);
-  const int nd = uniform<W>(P.op_ptr[qd + 1]) - od0; // 1 .. kStarMax
-  const int o1 = od0 + min(1, nd - 1);
-  const int l0 = uniform<W>(P.op_l[od0]), dl = uniform<W>(P.op_l[o1]) - l0;
-  const int u0 = uniform<W>(P.op_u[od0]), du = uniform<W>(P.op_u[o1]) - u0;
-  int y0 = 0, dy = 0;
-  if (row_with_diag) {
-    y0 = uniform<W>(P.lrow_col[r0]);
-    dy = uniform<W>(P.lrow_col[r0 + (o1 - od0)]) - y0;
-  }
-  double l[kStarMax], u[kStarMax], yv[kStarMax];
-#pragma unroll
-  for (int j = 0; j < kStarMax; ++j) {
-    const int k = min(j, nd - 1);
-    l[j] = LU[(size_t)(l0 + k * dl) * B + inst];
-    u[j] = A[(size_t)(u0 + k * du) * B + inst];
-    if (row_with_diag)
-      yv[j] = x[(size_t)(y0 + k * dy) * B + inst];
-  }
-  double v = A[(size_t)qd * B + inst];
-  double y = row_with_diag ? b[(size_t)c * B + inst] : 0.0;
-  // entries above the diagonal: no updates (LU = A), copied in chunks
-  for (int qb = q0; qb < qd; qb += 8) {
-    double t[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-      t[k] = A[(size_t)min(qb + k, qd - 1) * B + inst];
-#pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (qb + k < qd)
-        LU[(size_t)(qb + k) * B + inst] = t[k];
-  }
-#pragma unroll
-  for (int j = 0; j < kStarMax; ++j) {
-    const double pv = v - l[j] * u[j];
-    v = j < nd ? pv : v;
-    if (row_with_diag) {
-      const double py = y - l[j] * yv[j];
-      y = j < nd ? py : y;
-    }
-  }
-  if (fabs(v) < 1e-8)
-    record_status(P.status, PVGLU_E_SINGULAR, c, inst);
-  LU[(size_t)qd * B + inst] = v;
-  // entries below the diagonal: no updates, divided by the pivot
-  for (int qb = qd + 1; qb < q1; qb += 4) {
-    double t[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      t[k] = A[(size_t)min(qb + k, q1 - 1) * B + inst];
-#pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (qb + k < q1)
-        LU[(size_t)(qb + k) * B + inst] = t[k] / v;
-  }
-  if constexpr (FUSE) {
-    if (row_with_diag)
-      x[(size_t)c * B + inst] = y;
-    else
-      x[(size_t)c * B + inst] = dot_sub_chunked<W>(
-          b[(size_t)c * B + inst], P.lrow_slot, P.lrow_col, LU, x, r0,
-          uniform<W>(P.lrow_ptr[c + 1]), B, inst);
+    if (!row_with_diag)
+      yrow = dot_sub_chunked<W>(b[(size_t)c * B + inst], P.lrow_slot,
+                                P.lrow_col, LU, x, uniform<W>(P.lrow_ptr[c]),
+                                uniform<W>(P.lrow_ptr[c + 1]), B, inst);
+    x[(size_t)c * B + inst] = yrow;
+    yprev = yrow;
   }
 }
 
@@ -562,6 +482,113 @@ __device__ __forceinline__ void tri_lu_segment(
   p3 = 0.0;
 }
 
+// Star column (column flag bit 1; the DC bus): only the diagonal has
+// updates, so the entries above the diagonal are their A values, every U
+// operand of the diagonal is such an A value, and the entries below the
+// diagonal are A / pivot; the operands' slots and rows are arithmetic
+// progressions, addressed from their first two values (no per-operand index
+// loads).  All operands of the diagonal's sum (and of the fused forward row)
+// are loaded together, before anything of the column is stored, and the sums
+// run branch-free over a fixed register budget of kStarMax updates: the
+// column costs one memory round trip instead of one per operand chunk.  Used
+// by the narrow-run kernel, where the column is on a latency-bound chain.
+// Operation order per entry = refactor_column's.
+constexpr int kStarMax = 32;
+
+// The star column's 16-int record (built at plan time, at the column's
+// position in kernel order): q0, q1, diagonal slot, column flags, forward row
+// range r0 r1, number of updates, then (first, step) of the L slots, the U
+// slots and the forward rows.  One scalar load burst replaces the chain of
+// dependent metadata loads (column pointers -> update ranges -> operands).
+template <int W, bool FUSE>
+__device__ __forceinline__ void star_column(const Pat &P, int t, int c,
+                                            const double *A, double *LU,
+                                            const double *b, double *x,
+                                            size_t B, int inst, double &p0,
+                                            double &p1, double &p2,
+                                            double &p3, double &yprev) {
+  int f[16];
+  load_record<W>(P.lu_rec + (size_t)t * 16, f);
+  const int q0 = f[0], q1 = f[1], qd = f[2];
+  const bool row_with_diag = FUSE && (f[3] & 1) != 0;
+  const int nd = f[6]; // 1 .. kStarMax
+  const int l0 = f[7], dl = f[8], u0 = f[9], du = f[10], y0 = f[11],
+            dy = f[12];
+  double l[kStarMax], u[kStarMax], yv[kStarMax];
+#pragma unroll
+  for (int j = 0; j < kStarMax; ++j) {
+    const int k = min(j, nd - 1);
+    l[j] = LU[(size_t)(l0 + k * dl) * B + inst];
+    u[j] = A[(size_t)(u0 + k * du) * B + inst];
+    if (row_with_diag)
+      yv[j] = x[(size_t)(y0 + k * dy) * B + inst];
+  }
+  double v = A[(size_t)qd * B + inst];
+  double y = row_with_diag ? b[(size_t)c * B + inst] : 0.0;
+  // entries above the diagonal: no updates (LU = A), copied in chunks; the
+  // last four entries of the column are kept for the window
+  double w[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int qb = q0; qb < qd; qb += 8) {
+    double tv[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      tv[k] = A[(size_t)min(qb + k, qd - 1) * B + inst];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (qb + k < qd) {
+        LU[(size_t)(qb + k) * B + inst] = tv[k];
+        w[0] = w[1];
+        w[1] = w[2];
+        w[2] = w[3];
+        w[3] = tv[k];
+      }
+  }
+#pragma unroll
+  for (int j = 0; j < kStarMax; ++j) {
+    const double pv = v - l[j] * u[j];
+    v = j < nd ? pv : v;
+    if (row_with_diag) {
+      const double py = y - l[j] * yv[j];
+      y = j < nd ? py : y;
+    }
+  }
+  if (fabs(v) < 1e-8)
+    record_status(P.status, PVGLU_E_SINGULAR, c, inst);
+  LU[(size_t)qd * B + inst] = v;
+  w[0] = w[1];
+  w[1] = w[2];
+  w[2] = w[3];
+  w[3] = v;
+  // entries below the diagonal: no updates, divided by the pivot
+  for (int qb = qd + 1; qb < q1; qb += 4) {
+    double tv[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      tv[k] = A[(size_t)min(qb + k, q1 - 1) * B + inst];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+      if (qb + k < q1) {
+        const double lv = tv[k] / v;
+        LU[(size_t)(qb + k) * B + inst] = lv;
+        w[0] = w[1];
+        w[1] = w[2];
+        w[2] = w[3];
+        w[3] = lv;
+      }
+  }
+  p0 = w[0];
+  p1 = w[1];
+  p2 = w[2];
+  p3 = w[3];
+  if constexpr (FUSE) {
+    if (!row_with_diag)
+      y = dot_sub_chunked<W>(b[(size_t)c * B + inst], P.lrow_slot, P.lrow_col,
+                             LU, x, f[4], f[5], B, inst);
+    x[(size_t)c * B + inst] = y;
+    yprev = y;
+  }
+}
+
 // Backward-substitution values of one row: y, U(i,i) and the two U terms.
 struct RowVals {
   double y, dv, u0, u1;
@@ -598,12 +625,15 @@ __device__ __forceinline__ void pure_row_segment(const int (&f)[8], int len,
       });
 }
 
+// p0..p3 / yprev: the previous column's window (zero at a chain's start),
+// updated to this run's last column.
 template <int W, bool FUSE, int D = kLuDepth>
 __device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
                                               const double *A, double *LU,
                                               const double *b, double *x,
-                                              size_t B, int inst) {
-  double p0 = 0, p1 = 0, p2 = 0, p3 = 0, yprev = 0;
+                                              size_t B, int inst, double &p0,
+                                              double &p1, double &p2,
+                                              double &p3, double &yprev) {
   for (int t = t0; t < t1;) {
     const int *sg = P.lu_seg + (size_t)t * 4;
     const int len = uniform<W>(sg[0]);
@@ -792,15 +822,19 @@ __device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
   const int t1 = uniform<W>(L.chain_col_ptr[chain + 1]);
   if constexpr (OP == OP_REFACTOR_FUSED || OP == OP_REFACTOR) {
     constexpr bool FUSE = OP == OP_REFACTOR_FUSED;
+    double p0 = 0, p1 = 0, p2 = 0, p3 = 0, yprev = 0;
     if (uniform<W>(P.lu_fast[chain])) {
-      fast_lu_chain<W, FUSE, DL>(P, t0, t1, A, LU, b, x, B, inst);
+      fast_lu_chain<W, FUSE, DL>(P, t0, t1, A, LU, b, x, B, inst, p0, p1, p2,
+                                 p3, yprev);
       return;
     }
-    // mixed chain: register runs between generic columns
+    // mixed chain: register runs between generic columns; a run's first
+    // column may take operands from the generic column before it (window)
     for (int t = t0; t < t1;) {
       const int e = uniform<W>(P.lu_run[t]);
       if (e > t) {
-        fast_lu_chain<W, FUSE, DL>(P, t, e, A, LU, b, x, B, inst);
+        fast_lu_chain<W, FUSE, DL>(P, t, e, A, LU, b, x, B, inst, p0, p1, p2,
+                                   p3, yprev);
         t = e;
       } else {
         const int c = uniform<W>(L.cols[t]);
@@ -809,10 +843,12 @@ __device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
           star = (uniform<W>(P.drf[c]) & 2) != 0;
         if constexpr (NAR != 0) {
           if (star)
-            star_column<W, FUSE>(P, c, A, LU, b, x, B, inst);
+            star_column<W, FUSE>(P, t, c, A, LU, b, x, B, inst, p0, p1, p2,
+                                 p3, yprev);
         }
         if (!star)
-          refactor_column<W, FUSE>(P, c, A, LU, b, x, B, inst);
+          refactor_column<W, FUSE>(P, c, A, LU, b, x, B, inst, p0, p1, p2, p3,
+                                   yprev);
         ++t;
       }
     }
@@ -860,11 +896,7 @@ __device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
   }
   for (int t = t0; t < t1; ++t) {
     const int c = uniform<W>(L.cols[t]);
-    if constexpr (OP == OP_REFACTOR)
-      refactor_column<W, false>(P, c, A, LU, b, x, B, inst);
-    else if constexpr (OP == OP_REFACTOR_FUSED)
-      refactor_column<W, true>(P, c, A, LU, b, x, B, inst);
-    else if constexpr (OP == OP_LSOLVE)
+    if constexpr (OP == OP_LSOLVE)
       lsolve_row<W>(P, c, LU, b, x, B, inst);
     else if constexpr (OP == OP_USOLVE)
       usolve_row<W>(P, c, LU, x, B, inst);

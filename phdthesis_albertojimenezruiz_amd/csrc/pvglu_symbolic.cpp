@@ -268,12 +268,16 @@ static bool lu_column_qualifies(const pvglu_plan *P, int c) {
   return P->op_ptr[q0 + ne] - P->op_ptr[q0] <= kFastMaxOps;
 }
 
+// Operands in the previous column's register window are encoded as
+// -1 - (position in the window); the window is the previous column's last
+// four entries (all of them for a register column, which has at most four).
 static void encode_lu_record(const pvglu_plan *P, int c, int prev, int *r) {
   const int q0 = P->cp[c], ne = P->cp[c + 1] - q0;
   const int d = P->diag_slot[c] - q0;
+  const int w0 = prev >= 0 ? std::max(P->cp[prev], P->cp[prev + 1] - 4) : 0;
   auto lsrc_of = [&](int sl) {
-    if (prev >= 0 && sl >= P->cp[prev] && sl < P->cp[prev + 1])
-      return -1 - (sl - P->cp[prev]);
+    if (prev >= 0 && sl >= w0 && sl < P->cp[prev + 1])
+      return -1 - (sl - w0);
     return sl;
   };
   int nops = 0;
@@ -335,11 +339,13 @@ static void encode_u_record(const pvglu_plan *P, int i, int prev, int *r) {
 
 // Shared run splitting: positions of each chain are grouped into maximal runs
 // of qualifying columns; records are encoded with the previous column of the
-// run (none for a run's first column).
-template <class Q, class E>
+// run.  A run's first column is encoded against lead(previous column of the
+// chain) -- the column itself when the kernel leaves its register window
+// behind (a star column), else -1 (no forwarding).
+template <class Q, class E, class LEAD>
 static int compile_runs(const KernelLevels &K, int n, int rec_ints, Q qualifies,
                         E encode, std::vector<int> &rec, std::vector<int> &run,
-                        std::vector<int> &chain_flag) {
+                        std::vector<int> &chain_flag, LEAD lead) {
   const int nch = (int)K.chain_col_ptr.size() - 1;
   rec.assign((size_t)std::max(n, 1) * rec_ints, 0);
   run.assign(std::max(n, 1), 0);
@@ -357,7 +363,8 @@ static int compile_runs(const KernelLevels &K, int n, int rec_ints, Q qualifies,
       while (e < t1 && qualifies(K.cols[e]))
         ++e;
       for (int u = t; u < e; u++) {
-        encode(K.cols[u], u > t ? K.cols[u - 1] : -1,
+        encode(K.cols[u],
+               u > t ? K.cols[u - 1] : (u > t0 ? lead(K.cols[u - 1]) : -1),
                &rec[(size_t)u * rec_ints]);
         run[u] = e;
       }
@@ -413,17 +420,50 @@ static void compile_segments(const std::vector<int> &rec, int rec_ints,
 }
 
 void compile_fast_programs(pvglu_plan *P) {
+  auto none = [](int) { return -1; };
+  // refactor: a run after a star column forwards from its window (the star
+  // path and the generic gather both leave it in registers)
   P->fast_lu_chains = compile_runs(
       P->k_lu, P->n, 16, [&](int c) { return lu_column_qualifies(P, c); },
       [&](int c, int prev, int *r) { encode_lu_record(P, c, prev, r); },
-      P->fast_lu_rec, P->fast_lu_run, P->fast_lu_chain);
+      P->fast_lu_rec, P->fast_lu_run, P->fast_lu_chain,
+      [&](int c) { return (P->col_flags[c] & 2) ? c : -1; });
+  // star columns (never on the register path: more than four entries or
+  // forward terms) get their 16-int record at their position: q0, q1,
+  // diagonal slot, flags, forward row range, update count, then (first,
+  // step) of the L slots, U slots and forward rows
+  for (int t = 0; t < P->n; t++) {
+    const int c = P->k_lu.cols[t];
+    if (P->fast_lu_run[t] > t || !(P->col_flags[c] & 2))
+      continue;
+    const int qd = P->diag_slot[c];
+    const int o0 = P->op_ptr[qd], nd = P->op_ptr[qd + 1] - o0;
+    const int o1 = o0 + std::min(1, nd - 1);
+    const int r0 = P->lrow_ptr[c], r1 = P->lrow_ptr[c + 1];
+    int *r = &P->fast_lu_rec[(size_t)t * 16];
+    r[0] = P->cp[c];
+    r[1] = P->cp[c + 1];
+    r[2] = qd;
+    r[3] = P->col_flags[c];
+    r[4] = r0;
+    r[5] = r1;
+    r[6] = nd;
+    r[7] = P->op_l[o0];
+    r[8] = P->op_l[o1] - P->op_l[o0];
+    r[9] = P->op_u[o0];
+    r[10] = P->op_u[o1] - P->op_u[o0];
+    if (P->col_flags[c] & 1) {
+      r[11] = P->lrow_col[r0];
+      r[12] = P->lrow_col[r0 + (o1 - o0)] - P->lrow_col[r0];
+    }
+  }
   P->fast_u_chains = compile_runs(
       P->k_u, P->n, 8,
       [&](int i) {
         return P->urow_ptr[i + 1] - P->urow_ptr[i] <= kFastMaxLr;
       },
       [&](int i, int prev, int *r) { encode_u_record(P, i, prev, r); },
-      P->fast_u_rec, P->fast_u_run, P->fast_u_chain);
+      P->fast_u_rec, P->fast_u_run, P->fast_u_chain, none);
   // seg[4t+3] bit 1: every update operand of the record comes from the
   // previous column's registers; bit 0: so do the forward-substitution
   // operands.  Such a segment issues no pattern-addressed loads: the kernel
@@ -462,21 +502,21 @@ void compile_fast_programs(pvglu_plan *P) {
         return P->lrow_ptr[i + 1] - P->lrow_ptr[i] <= kFastMaxLr;
       },
       [&](int i, int prev, int *r) { encode_l_record(P, i, prev, r); },
-      P->fast_l_rec, P->fast_l_run, P->fast_l_chain);
+      P->fast_l_rec, P->fast_l_run, P->fast_l_chain, none);
   compile_segments(P->fast_l_rec, 8, P->fast_l_run, P->n, P->fast_l_seg,
                    row_forwarded);
   P->fast_ut_chains = compile_runs(
       P->k_lu, P->n, 8,
       [&](int j) { return P->diag_slot[j] - P->cp[j] <= kFastMaxLr; },
       [&](int j, int prev, int *r) { encode_t_record(P, j, prev, true, r); },
-      P->fast_ut_rec, P->fast_ut_run, P->fast_ut_chain);
+      P->fast_ut_rec, P->fast_ut_run, P->fast_ut_chain, none);
   compile_segments(P->fast_ut_rec, 8, P->fast_ut_run, P->n, P->fast_ut_seg,
                    row_forwarded);
   P->fast_lt_chains = compile_runs(
       P->k_lt, P->n, 8,
       [&](int j) { return P->cp[j + 1] - P->diag_slot[j] - 1 <= kFastMaxLr; },
       [&](int j, int prev, int *r) { encode_t_record(P, j, prev, false, r); },
-      P->fast_lt_rec, P->fast_lt_run, P->fast_lt_chain);
+      P->fast_lt_rec, P->fast_lt_run, P->fast_lt_chain, none);
   compile_segments(P->fast_lt_rec, 8, P->fast_lt_run, P->n, P->fast_lt_seg,
                    row_forwarded);
   P->fast_lu_segs = P->fast_u_segs = 0;

@@ -141,10 +141,12 @@ def test_star_columns_and_tridiagonal_segments_of_the_plant():
     starts = seg[seg[:, 0] > 0]
     tri = starts[(starts[:, 3] & 12) == 12]
     # forwarded operand: entry 1 of the bottom column (diagonal, L), entry 2
-    # of a tridiagonal column (U, diagonal, L)
-    assert set(((tri[:, 3] >> 4) & 3).tolist()) == {1, 2}
+    # of a tridiagonal column (U, diagonal, L), entry 3 of the bus column's
+    # register window (its last four entries, the L entry last)
+    k = (tri[:, 3] >> 4) & 3
+    assert set(k.tolist()) == {1, 2, 3}
+    assert np.count_nonzero(k == 3) == I
     # every string column but the bottom one is a tridiagonal step, except
-    # two per inverter whose forwarded operand is in memory: the column after
-    # the bus (a star column, outside the register path) and the last
+    # one per inverter whose forwarded operand is in memory: the last
     # string's column after its bottom node (eliminated a level earlier)
-    assert tri[:, 0].sum() == I * S * (M - 1) - 2 * I
+    assert tri[:, 0].sum() == I * S * (M - 1) - I
