@@ -401,6 +401,87 @@ static void launch_one_level(pvglu_plan *P, const DevLevels &L, int first,
 // Chains per component at or below which a level counts as narrow.
 constexpr int kNarrowChains = 4;
 
+// The launches of one numeric call (mode 0 refactor, 1 refactor + solve,
+// 2 solve, 3 transposed solve) under LEVEL (hybrid = false) or HYBRID: a
+// level launch of level lv of level set `set` (0 LU, 1 L, 2 U, 3 L^T), or a
+// narrow-run launch over the level ranges R (HYBRID merges each run of two
+// or more consecutive narrow steps; `widest` = most chains per component in
+// it).
+struct Item {
+  bool run;
+  int set, lv, op, widest;
+  LvRange R;
+};
+
+static void mode_ops(const pvglu_plan *P, int mode, int ops[4]) {
+  for (int k = 0; k < 4; ++k)
+    ops[k] = -1;
+  const bool fused = mode == 1 && P->fuse_lsolve;
+  if (mode == 3) {
+    ops[0] = OP_UT;
+    ops[3] = OP_LT;
+  } else {
+    if (mode <= 1)
+      ops[0] = fused ? OP_REFACTOR_FUSED : OP_REFACTOR;
+    if (mode == 2 || (mode == 1 && !fused))
+      ops[1] = OP_LSOLVE;
+    if (mode >= 1)
+      ops[2] = OP_USOLVE;
+  }
+}
+
+static std::vector<Item> build_items(const pvglu_plan *P, int mode,
+                                     bool hybrid) {
+  const DevLevels *sets[4] = {&P->dev->lu, &P->dev->l, &P->dev->u,
+                              &P->dev->lt};
+  int ops[4];
+  mode_ops(P, mode, ops);
+  struct Step {
+    int set, lv, width;
+  };
+  std::vector<Step> steps;
+  for (int k = 0; k < 4; ++k)
+    if (ops[k] >= 0)
+      for (int lv = 0; lv < sets[k]->nlev; ++lv)
+        steps.push_back({k, lv, sets[k]->h_level_comp_chains[lv]});
+  std::vector<Item> items;
+  const int nsteps = (int)steps.size();
+  for (int i = 0; i < nsteps;) {
+    int j = i, widest = 1;
+    if (hybrid)
+      while (j < nsteps && steps[j].width <= kNarrowChains) {
+        widest = std::max(widest, steps[j].width);
+        ++j;
+      }
+    Item it{};
+    if (j - i < 2) {
+      it.run = false;
+      it.set = steps[i].set;
+      it.lv = steps[i].lv;
+      it.op = ops[it.set];
+      items.push_back(it);
+      ++i;
+      continue;
+    }
+    it.run = true;
+    it.widest = widest;
+    for (int k = 0; k < 4; ++k) {
+      it.R.lo[k] = 0;
+      it.R.hi[k] = 0;
+      it.R.op[k] = ops[k] < 0 ? 0 : ops[k];
+    }
+    for (int t = i; t < j; ++t) {
+      const Step &sp = steps[t];
+      if (it.R.lo[sp.set] == it.R.hi[sp.set])
+        it.R.lo[sp.set] = sp.lv;
+      it.R.hi[sp.set] = sp.lv + 1;
+    }
+    items.push_back(it);
+    i = j;
+  }
+  return items;
+}
+
 static int group_index(int G) { return G >= 4 ? 2 : (G == 2 ? 1 : 0); }
 
 // LEVEL and HYBRID launches.  The independent components are split into G
@@ -430,28 +511,7 @@ static int launch_steps_w(pvglu_plan *P, int mode, bool hybrid, int G,
   const int gi = group_index(G);
   G = 1 << gi;
   const DevLevels *sets[4] = {&d->lu, &d->l, &d->u, &d->lt};
-  int ops[4] = {-1, -1, -1, -1};
-  const bool fused = mode == 1 && P->fuse_lsolve;
-  if (mode == 3) {
-    ops[0] = OP_UT;
-    ops[3] = OP_LT;
-  } else {
-    if (mode <= 1)
-      ops[0] = fused ? OP_REFACTOR_FUSED : OP_REFACTOR;
-    if (mode == 2 || (mode == 1 && !fused))
-      ops[1] = OP_LSOLVE;
-    if (mode >= 1)
-      ops[2] = OP_USOLVE;
-  }
-  struct Step {
-    int set, lv, width;
-  };
-  std::vector<Step> steps;
-  for (int k = 0; k < 4; ++k)
-    if (ops[k] >= 0)
-      for (int lv = 0; lv < sets[k]->nlev; ++lv)
-        steps.push_back({k, lv, sets[k]->h_level_comp_chains[lv]});
-  const int nsteps = (int)steps.size();
+  const std::vector<Item> items = build_items(P, mode, hybrid);
   const int tiles = (B + W - 1) / W;
   const Pat pat = pat_of(d);
   if (G > 1) {
@@ -465,41 +525,20 @@ static int launch_steps_w(pvglu_plan *P, int mode, bool hybrid, int G,
       HIP_TRY(hipStreamWaitEvent(st, d->ev_skew[g - 1], 0));
     const int c0 = d->comp_bounds[gi][g], c1 = d->comp_bounds[gi][g + 1];
     bool skewed = false;
-    for (int i = 0; i < nsteps;) {
-      int j = i, widest = 1;
-      if (hybrid)
-        while (j < nsteps && steps[j].width <= kNarrowChains) {
-          widest = std::max(widest, steps[j].width);
-          ++j;
-        }
-      if (j - i < 2) {
-        const Step &sp = steps[i];
-        const DevLevels &L = *sets[sp.set];
-        const int *r = &L.h_grp[gi][((size_t)g * L.nlev + sp.lv) * 2];
-        launch_one_level<W>(P, L, r[0], r[1] - r[0], ops[sp.set], A, LU, b, x,
-                            B, st);
-        ++i;
+    for (const Item &it : items) {
+      if (!it.run) {
+        const DevLevels &L = *sets[it.set];
+        const int *r = &L.h_grp[gi][((size_t)g * L.nlev + it.lv) * 2];
+        launch_one_level<W>(P, L, r[0], r[1] - r[0], it.op, A, LU, b, x, B,
+                            st);
       } else {
-        LvRange R;
-        for (int k = 0; k < 4; ++k) {
-          R.lo[k] = 0;
-          R.hi[k] = 0;
-          R.op[k] = ops[k] < 0 ? 0 : ops[k];
-        }
-        for (int t = i; t < j; ++t) {
-          const Step &sp = steps[t];
-          if (R.lo[sp.set] == R.hi[sp.set])
-            R.lo[sp.set] = sp.lv;
-          R.hi[sp.set] = sp.lv + 1;
-        }
         int slots = 1;
-        while (slots < widest)
+        while (slots < it.widest)
           slots *= 2;
         const int threads = std::max(64, slots * W);
         Launch<W>::hybrid(pat, lev_of(d->lu), lev_of(d->l), lev_of(d->u),
-                          lev_of(d->lt), R, c0, c1 - c0, tiles, threads, A,
+                          lev_of(d->lt), it.R, c0, c1 - c0, tiles, threads, A,
                           LU, b, x, B, st);
-        i = j;
       }
       if (g + 1 < G && !skewed) {
         // the next group starts once this group's first step is done
@@ -849,6 +888,11 @@ struct pvglu_multi {
   std::vector<pvglu_plan *> plans;
   std::vector<int> batches;
   int width = 16, threads = 256;
+  // multi-pattern HYBRID steps, per mode (1 refactor + solve, 3 transposed
+  // solve): usable when every group's step sequence has the same shape
+  bool steps_ok[4] = {false, false, false, false};
+  int step_width = 64;
+  std::vector<std::vector<Item>> items[4];
 };
 
 extern "C" int pvglu_multi_create(int ngroups, pvglu_plan *const *plans,
@@ -881,6 +925,23 @@ extern "C" int pvglu_multi_create(int ngroups, pvglu_plan *const *plans,
   while (slots < maxc && slots * 2 * m->width <= 1024)
     slots *= 2;
   m->threads = std::min(1024, std::max(64, slots * m->width));
+  // HYBRID steps across the groups: one wide-level launch per wide level and
+  // one narrow-run launch per run, each covering every group, when the step
+  // sequences agree (same topology at different sizes) and the batches fill
+  // at least 16 lanes; otherwise the component-style work list above
+  m->step_width = minB >= 32 ? 64 : 16;
+  for (int mode : {1, 3}) {
+    bool ok = minB >= 16;
+    for (int g = 0; g < ngroups && ok; g++) {
+      m->items[mode].push_back(build_items(plans[g], mode, true));
+      const auto &a = m->items[mode][0], &b = m->items[mode][g];
+      ok = a.size() == b.size();
+      for (size_t i = 0; ok && i < a.size(); i++)
+        ok = a[i].run == b[i].run &&
+             (a[i].run || (a[i].set == b[i].set && a[i].op == b[i].op));
+    }
+    m->steps_ok[mode] = ok;
+  }
   *out = m;
   return PVGLU_OK;
 }
@@ -919,6 +980,78 @@ static int launch_multi_w(pvglu_multi *m, int mode, const double *const *a,
   return PVGLU_OK;
 }
 
+template <int W>
+static int launch_multi_steps_w(pvglu_multi *m, int mode,
+                                const double *const *a, double *const *lu,
+                                const double *const *b, double *const *x,
+                                hipStream_t s) {
+  const int ng = m->ngroups;
+  const size_t nitems = m->items[mode][0].size();
+  for (size_t i = 0; i < nitems; i++) {
+    if (!m->items[mode][0][i].run) {
+      MultiLevel ml{};
+      ml.ngroups = ng;
+      int blocks = 0;
+      for (int g = 0; g < ng; g++) {
+        const pvglu_plan *P = m->plans[g];
+        const DevicePlan *d = P->dev;
+        const Item &it = m->items[mode][g][i];
+        const DevLevels *sets[4] = {&d->lu, &d->l, &d->u, &d->lt};
+        const DevLevels &L = *sets[it.set];
+        const int first = L.h_level_chain_ptr[it.lv];
+        const int nch = L.h_level_chain_ptr[it.lv + 1] - first;
+        const int tiles = (m->batches[g] + W - 1) / W;
+        ml.block_start[g] = blocks;
+        blocks += (int)(((long)nch * tiles * W + 255) / 256);
+        ml.first[g] = first;
+        ml.nch[g] = nch;
+        ml.tiles[g] = tiles;
+        ml.B[g] = m->batches[g];
+        ml.pat[g] = pat_of(d);
+        ml.lev[g] = lev_of(L);
+        ml.a[g] = a ? a[g] : nullptr;
+        ml.luv[g] = lu[g];
+        ml.b[g] = b[g];
+        ml.x[g] = x[g];
+      }
+      ml.block_start[ng] = blocks;
+      Launch<W>::level_multi(ml, m->items[mode][0][i].op, blocks, s);
+    } else {
+      MultiRun mr{};
+      mr.ngroups = ng;
+      int blocks = 0, threads = 64;
+      for (int g = 0; g < ng; g++) {
+        const pvglu_plan *P = m->plans[g];
+        const DevicePlan *d = P->dev;
+        const Item &it = m->items[mode][g][i];
+        int slots = 1;
+        while (slots < it.widest)
+          slots *= 2;
+        threads = std::max(threads, slots * W);
+        const int tiles = (m->batches[g] + W - 1) / W;
+        mr.block_start[g] = blocks;
+        blocks += P->ncomp * tiles;
+        mr.tiles[g] = tiles;
+        mr.B[g] = m->batches[g];
+        mr.R[g] = it.R;
+        mr.pat[g] = pat_of(d);
+        mr.lu[g] = lev_of(d->lu);
+        mr.l[g] = lev_of(d->l);
+        mr.u[g] = lev_of(d->u);
+        mr.lt[g] = lev_of(d->lt);
+        mr.a[g] = a ? a[g] : nullptr;
+        mr.luv[g] = lu[g];
+        mr.b[g] = b[g];
+        mr.x[g] = x[g];
+      }
+      mr.block_start[ng] = blocks;
+      Launch<W>::run_multi(mr, blocks, threads, s);
+    }
+  }
+  HIP_TRY(hipGetLastError());
+  return PVGLU_OK;
+}
+
 static int launch_multi(pvglu_multi *m, int mode, const double *const *a,
                         double *const *lu, const double *const *b,
                         double *const *x, void *stream) {
@@ -929,6 +1062,10 @@ static int launch_multi(pvglu_multi *m, int mode, const double *const *a,
     if (!lu[g] || !b[g] || !x[g] || (mode != 3 && !a[g]))
       return fail(PVGLU_E_ARG, "multi: null device pointer in group %d", g);
   hipStream_t s = (hipStream_t)stream;
+  if (m->steps_ok[mode])
+    return m->step_width == 64
+               ? launch_multi_steps_w<64>(m, mode, a, lu, b, x, s)
+               : launch_multi_steps_w<16>(m, mode, a, lu, b, x, s);
   switch (m->width) {
   case 1: return launch_multi_w<1>(m, mode, a, lu, b, x, s);
   case 4: return launch_multi_w<4>(m, mode, a, lu, b, x, s);

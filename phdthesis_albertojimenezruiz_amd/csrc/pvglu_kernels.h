@@ -1049,6 +1049,59 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Multi-pattern LEVEL step: group g's blocks take its chains of the level
+// as k_level does (backward sweeps in descending order).
+template <int W, int OP>
+__global__ void __launch_bounds__(256) k_level_multi(MultiLevel m) {
+  int g = 0;
+  while (g + 1 < m.ngroups && (int)blockIdx.x >= m.block_start[g + 1])
+    ++g;
+  const int gid = ((int)blockIdx.x - m.block_start[g]) * blockDim.x + threadIdx.x;
+  const int g0 = uniform<W>(gid / W);
+  const int lane = gid % W;
+  const int tiles = m.tiles[g], total = m.nch[g] * tiles;
+  if (g0 >= total)
+    return;
+  constexpr bool backward = OP == OP_USOLVE || OP == OP_LT;
+  const int group = backward ? total - 1 - g0 : g0;
+  const int tile = group % tiles;
+  const int chain = m.first[g] + group / tiles;
+  const int inst = tile * W + lane;
+  if (inst >= m.B[g])
+    return;
+  run_chain<W, OP>(m.pat[g], m.lev[g], chain, m.a[g], m.luv[g], m.b[g],
+                   m.x[g], (size_t)m.B[g], inst);
+}
+
+// Multi-pattern narrow run: block -> (group, component, tile), the body of
+// k_hybrid with the group's plan and level ranges.
+template <int W>
+__global__ void __launch_bounds__(256) k_run_multi(MultiRun m) {
+  int g = 0;
+  while (g + 1 < m.ngroups && (int)blockIdx.x >= m.block_start[g + 1])
+    ++g;
+  const int local = (int)blockIdx.x - m.block_start[g];
+  const int tiles = m.tiles[g];
+  const int comp = local / tiles, tile = local % tiles;
+  const int slots = blockDim.x / W;
+  const int slot = uniform<W>((int)threadIdx.x / W);
+  const int inst = tile * W + (int)threadIdx.x % W;
+  const int B = m.B[g];
+  const bool active = inst < B;
+  const LvRange &R = m.R[g];
+  for (int k = 0; k < 4; ++k) {
+    const int lo = k == 0 ? R.lo[0] : k == 1 ? R.lo[1] : k == 2 ? R.lo[2] : R.lo[3];
+    const int hi = k == 0 ? R.hi[0] : k == 1 ? R.hi[1] : k == 2 ? R.hi[2] : R.hi[3];
+    const int op = k == 0 ? R.op[0] : k == 1 ? R.op[1] : k == 2 ? R.op[2] : R.op[3];
+    if (lo >= hi)
+      continue;
+    const Lev Lk = k == 0 ? m.lu[g] : k == 1 ? m.l[g] : k == 2 ? m.u[g] : m.lt[g];
+    component_levels<W, 1>(m.pat[g], Lk, comp, slot, slots, active, m.a[g],
+                           m.luv[g], m.b[g], m.x[g], (size_t)B, inst, op, lo,
+                           hi);
+  }
+}
+
 // Multi-pattern work list: groups of instances with their own plans in one
 // launch; block -> (group, component, instance tile).
 
@@ -1110,6 +1163,31 @@ template <int W>
 void Launch<W>::multi(const MultiArgs &m, int mode, int blocks, int threads,
                       hipStream_t s) {
   hipLaunchKernelGGL((k_multi<W>), dim3(blocks), dim3(threads), 0, s, m, mode);
+}
+
+template <int W>
+void Launch<W>::level_multi(const MultiLevel &m, int op, int blocks,
+                            hipStream_t s) {
+  if (blocks == 0)
+    return;
+#define PVGLU_LVLM(OPC)                                                        \
+  hipLaunchKernelGGL((k_level_multi<W, OPC>), dim3(blocks), dim3(256), 0, s, m)
+  switch (op) {
+  case OP_REFACTOR: PVGLU_LVLM(OP_REFACTOR); break;
+  case OP_REFACTOR_FUSED: PVGLU_LVLM(OP_REFACTOR_FUSED); break;
+  case OP_LSOLVE: PVGLU_LVLM(OP_LSOLVE); break;
+  case OP_USOLVE: PVGLU_LVLM(OP_USOLVE); break;
+  case OP_UT: PVGLU_LVLM(OP_UT); break;
+  default: PVGLU_LVLM(OP_LT); break;
+  }
+#undef PVGLU_LVLM
+}
+
+template <int W>
+void Launch<W>::run_multi(const MultiRun &m, int blocks, int threads,
+                          hipStream_t s) {
+  if (blocks > 0)
+    hipLaunchKernelGGL((k_run_multi<W>), dim3(blocks), dim3(threads), 0, s, m);
 }
 
 } // namespace pvglu

@@ -96,6 +96,36 @@ __device__ __forceinline__ void record_status(int *status, int code, int col,
   }
 }
 
+// Multi-pattern LEVEL / HYBRID steps: one launch covers the same step of
+// every group (plans of one topology at different sizes, so their step
+// sequences agree); blocks [block_start[g], block_start[g + 1]) belong to
+// group g.  MultiLevel: one level (chains [first, first + nch) of a level
+// set, operation op); MultiRun: a narrow run, block -> (component, tile).
+struct MultiLevel {
+  int ngroups;
+  int block_start[kMaxGroups + 1];
+  int first[kMaxGroups], nch[kMaxGroups], tiles[kMaxGroups], B[kMaxGroups];
+  Pat pat[kMaxGroups];
+  Lev lev[kMaxGroups];
+  const double *a[kMaxGroups];
+  double *luv[kMaxGroups];
+  const double *b[kMaxGroups];
+  double *x[kMaxGroups];
+};
+
+struct MultiRun {
+  int ngroups;
+  int block_start[kMaxGroups + 1];
+  int tiles[kMaxGroups], B[kMaxGroups];
+  LvRange R[kMaxGroups];
+  Pat pat[kMaxGroups];
+  Lev lu[kMaxGroups], l[kMaxGroups], u[kMaxGroups], lt[kMaxGroups];
+  const double *a[kMaxGroups];
+  double *luv[kMaxGroups];
+  const double *b[kMaxGroups];
+  double *x[kMaxGroups];
+};
+
 // Launchers of the numeric kernels at instance width W.  All are stream
 // ordered and neither synchronize nor allocate.
 template <int W> struct Launch {
@@ -119,6 +149,12 @@ template <int W> struct Launch {
   // multi-pattern work list
   static void multi(const MultiArgs &m, int mode, int blocks, int threads,
                     hipStream_t s);
+  // multi-pattern LEVEL / HYBRID steps (256-thread blocks for levels,
+  // `threads` <= 256 for runs)
+  static void level_multi(const MultiLevel &m, int op, int blocks,
+                          hipStream_t s);
+  static void run_multi(const MultiRun &m, int blocks, int threads,
+                        hipStream_t s);
 };
 
 } // namespace pvglu

@@ -366,12 +366,28 @@ def test_device_inclusive_scan(dev, m):
     assert np.array_equal(d.cpu().numpy(), np.cumsum(a).astype(np.int32))
 
 
-def test_multi_pattern_batch(dev):
+# (inverters, batch) per group; the smallest batch decides the path: below
+# 16 the component-style work list, 16..31 HYBRID steps at width 16, else at
+# width 64; a random pattern next to a plant has another step sequence and
+# takes the work list whatever the batch
+MULTI_CASES = [((1, 7), (2, 20), (5, 3)), ((1, 17), (2, 20), (5, 30)),
+               ((1, 40), (2, 33), (5, 64)), ((2, 20), ("random", 24))]
+
+
+@pytest.mark.parametrize("case", range(len(MULTI_CASES)))
+def test_multi_pattern_batch(dev, case):
     """Mixed plant sizes (cfg3 shape, scaled: 1/2/5 inverters x 24 x 28) in
-    one launch, forward and transposed, bit-exact per instance."""
+    one launch per step, forward and transposed, bit-exact per instance."""
     rng = np.random.default_rng(77)
     groups = []
-    for I, B in ((1, 7), (2, 20), (5, 3)):
+    for I, B in MULTI_CASES[case]:
+        if I == "random":
+            _, n, rp, ci = _random_case(5, n=90)
+            s = O.csr_csc(n, rp, ci)
+            A = np.stack([dd_values(rng, n, rp, ci)[s.csc2csr] for _ in range(B)], axis=1)
+            bv = rng.normal(size=(n, B))
+            groups.append((pv.Plan(n, rp, ci), s, A, bv, B))
+            continue
         n, rows = plant_pattern(I, 24, 28)
         rp, ci = csr_from_sets(rows)
         plan = pv.Plan(n, rp, ci)
