@@ -166,7 +166,10 @@ int pvglu_plan_load(const char *path, pvglu_plan **out, int *P, int *Prow);
  *   PVGLU_SCHED_HYBRID     wide levels one kernel each; every run of two or
  *                          more consecutive narrow levels (at most 4 chains
  *                          per component) one COMPONENT-style launch
- * width: instances handled side by side per chain (1..64, power of two). */
+ *   PVGLU_SCHED_AUTO       HYBRID from batch 16 up, COMPONENT below (for
+ *                          plans with 8 or more independent components)
+ * width: instances handled side by side per chain (1..64, power of two;
+ * 0: the largest of 64, 32, 16, 4, 1 not above the batch). */
 enum {
   PVGLU_SCHED_AUTO = 0,
   PVGLU_SCHED_LEVEL = 1,

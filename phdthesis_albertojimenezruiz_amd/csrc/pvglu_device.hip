@@ -365,11 +365,14 @@ static void pick_default_schedule(pvglu_plan *P, int B, int &variant, int &W,
                                   int &threads) {
   variant = P->sched_variant;
   W = P->sched_width;
+  // measured on MI355X: HYBRID wins from batch 16 up (40x24x28 plant:
+  // batch 64 0.061 ms vs COMPONENT 0.110; 400x24x28, batch 32: 0.198 vs
+  // 0.282 ms), COMPONENT below (batch 1: 0.050 vs 0.091 ms)
   if (!valid_width(W))
-    W = B >= 64 ? 64 : (B >= 16 ? 16 : (B >= 4 ? 4 : 1));
+    W = B >= 64 ? 64 : (B >= 32 ? 32 : (B >= 16 ? 16 : (B >= 4 ? 4 : 1)));
   if (variant == PVGLU_SCHED_AUTO)
-    variant = (P->ncomp >= 8 && B < 256) ? PVGLU_SCHED_COMPONENT
-                                         : PVGLU_SCHED_HYBRID;
+    variant = (B < 16 && P->ncomp >= 8) ? PVGLU_SCHED_COMPONENT
+                                        : PVGLU_SCHED_HYBRID;
   // COMPONENT: chain slots per block = widest component level, capped so the
   // block has at most 1024 threads.
   int maxc = std::max({P->k_lu.max_comp_chains, P->k_u.max_comp_chains,
