@@ -441,14 +441,19 @@ __device__ __forceinline__ void pure_lu_segment(
 // entry k times the U entry is the pivot, the L entry is divided by it, and
 // the fused forward row subtracts the same L operand times the previous y.
 // The same operations as pure_lu_col on such a record, without its
-// run-time record decoding (selects and branches per update).
-template <bool FUSE, int D>
+// run-time record decoding (selects and branches per update); KC = 2 fixes
+// the forwarded entry at compile time (the string columns), and small
+// pivots are looked for once per segment (running minimum, then a rescan
+// of the stored pivots in the rare case).
+template <bool FUSE, int D, int KC = -1>
 __device__ __forceinline__ void tri_lu_segment(
     int slot0, int col0, int len, int dslot, int dcol, int k, const double *A,
     double *LU, const double *b, double *x, size_t B, int inst, double &p0,
     double &p1, double &p2, double &p3, double &yprev, int &badcol) {
+  if (KC >= 0)
+    k = KC;
   double pl = sel4(p0, p1, p2, p3, k);
-  double v0 = 0.0, v1 = 0.0, v2 = 0.0;
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, minpiv = 1.0;
   pipeline<D, ColVals>(
       len,
       [&](ColVals &c, int i) {
@@ -460,8 +465,7 @@ __device__ __forceinline__ void tri_lu_segment(
         const size_t s = (size_t)(slot0 + i * dslot) * B + inst;
         const double u = c.a0;
         const double piv = c.a1 - pl * u;
-        if (fabs(piv) < 1e-8 && badcol < 0)
-          badcol = col0 + i * dcol;
+        minpiv = fmin(minpiv, fabs(piv));
         const double lo = c.a2 / piv;
         LU[s] = u;
         LU[s + B] = piv;
@@ -474,8 +478,17 @@ __device__ __forceinline__ void tri_lu_segment(
         v0 = u;
         v1 = piv;
         v2 = lo;
-        pl = k == 0 ? u : (k == 1 ? piv : (k == 2 ? lo : 0.0));
+        if constexpr (KC == 2)
+          pl = lo;
+        else
+          pl = k == 0 ? u : (k == 1 ? piv : (k == 2 ? lo : 0.0));
       });
+  if (minpiv < 1e-8) // rare: find the first small pivot of the segment
+    for (int i = 0; i < len; ++i)
+      if (fabs(LU[(size_t)(slot0 + i * dslot + 1) * B + inst]) < 1e-8) {
+        badcol = col0 + i * dcol;
+        break;
+      }
   p0 = v0;
   p1 = v1;
   p2 = v2;
@@ -647,9 +660,13 @@ __device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
     const int sflag = uniform<W>(sg[3]);
     if (sflag & (FUSE ? 8 : 4)) {
       int badcol = -1;
-      tri_lu_segment<FUSE, D>(f[0], f[1], len, dslot, dcol, (sflag >> 4) & 3,
-                              A, LU, b, x, B, inst, p0, p1, p2, p3, yprev,
-                              badcol);
+      if (((sflag >> 4) & 3) == 2)
+        tri_lu_segment<FUSE, D, 2>(f[0], f[1], len, dslot, dcol, 2, A, LU, b,
+                                   x, B, inst, p0, p1, p2, p3, yprev, badcol);
+      else
+        tri_lu_segment<FUSE, D>(f[0], f[1], len, dslot, dcol,
+                                (sflag >> 4) & 3, A, LU, b, x, B, inst, p0, p1,
+                                p2, p3, yprev, badcol);
       if (badcol >= 0)
         record_status(P.status, PVGLU_E_SINGULAR, badcol, inst);
       t += len;

@@ -272,6 +272,37 @@ def test_star_and_tridiagonal_paths_bitexact(dev, case, sched, B):
     assert torch.equal(dlu2, dlu) and torch.equal(dx2, dx)
 
 
+@pytest.mark.parametrize("sched", [(pv.SCHED_HYBRID, 64), (pv.SCHED_LEVEL, 64),
+                                   (pv.SCHED_COMPONENT, 16), (pv.SCHED_LEVEL, 1)])
+def test_singular_pivot_in_a_string_segment(dev, sched):
+    """Two pivots below 1e-8 in one string (columns 3 and 4 of the first
+    string, instance 37 of 70): the first is reported, whichever kernel path
+    (tridiagonal segment, generic gather) the schedule takes."""
+    n, rp, ci = _hub_case([6, 6, 6], 5)
+    rng = np.random.default_rng(8)
+    plan = pv.Plan(n, rp, ci)
+    plan.set_schedule(*sched)
+    s = O.csr_csc(n, rp, ci)
+    B = 70
+    A = np.stack([dd_values(rng, n, rp, ci)[s.csc2csr] for _ in range(B)], axis=1)
+    for c in (3, 4):
+        q0, qd = s.cp[c], s.csc_diag[c] - 1            # U entry (c-1, c), pivot
+        assert qd == q0 + 1
+        A[q0, 37] = 0.0                                  # pivot = its A value
+        A[qd, 37] = 1e-9
+    dA = _to_dev(A, dev)
+    plan.status_reset()
+    plan.refactor(dA, dA, B)
+    torch.cuda.synchronize()
+    assert plan.status() == (pv.E_SINGULAR, 3, 37)
+    db = _to_dev(rng.normal(size=(n, B)), dev)
+    dA = _to_dev(A, dev)
+    plan.status_reset()
+    plan.refactor_solve(dA, torch.empty_like(dA), db, torch.empty_like(db), B)
+    torch.cuda.synchronize()
+    assert plan.status() == (pv.E_SINGULAR, 3, 37)
+
+
 def test_singular_pivot_is_reported(dev):
     rng, n, rp, ci = _random_case(3, n=40)
     plan = pv.Plan(n, rp, ci)
