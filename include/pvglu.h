@@ -188,6 +188,20 @@ int pvglu_get_schedule(const pvglu_plan *plan, int *variant, int *width);
  * numeric calls on one plan from different host threads at the same time
  * need G = 1.  No counterpart in the reference. */
 int pvglu_set_stream_groups(pvglu_plan *plan, int groups);
+
+/* Cache policy of the width-64 LEVEL and HYBRID kernels' value streams.
+ * STREAMING marks the loads of A, b and the substitution operands and the
+ * stores of L entries and forward results non-temporal (they are not read
+ * again before eviction); DEFAULT leaves every access to the cache; AUTO
+ * (default) streams when the batch's algorithmic bytes exceed twice the
+ * 256 MB Infinity Cache.  pvglu_autotune tries both.  No counterpart in the
+ * reference. */
+enum {
+  PVGLU_CACHE_AUTO = 0,
+  PVGLU_CACHE_STREAMING = 1,
+  PVGLU_CACHE_DEFAULT = 2
+};
+int pvglu_set_cache_policy(pvglu_plan *plan, int policy);
 int pvglu_get_stream_groups(const pvglu_plan *plan);
 
 /* Times the schedule candidates (variant x width x component groups) for this

@@ -24,6 +24,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpvglu.so
 
 OK, E_SINGULAR, E_NODIAG, E_HIP, E_ARG, E_NOCONV, E_STATE = 0, -1, -2, -3, -4, -5, -6
 SCHED_AUTO, SCHED_LEVEL, SCHED_COMPONENT, SCHED_HYBRID = 0, 1, 2, 3
+CACHE_AUTO, CACHE_STREAMING, CACHE_DEFAULT = 0, 1, 2
 
 INFO = dict(n=0, nnz=1, nlev_lu=2, nchain_lu=3, nlev_l=4, nchain_l=5, nlev_u=6,
             nchain_u=7, nlev_lt=8, nchain_lt=9, nops=10, fused_lsolve=11, ncomp=12,
@@ -75,6 +76,7 @@ def lib():
         "pvglu_plan_file_n": (C.c_int, [C.c_char_p]),
         "pvglu_plan_load": (C.c_int, [C.c_char_p, C.POINTER(_VP), _P_INT, _P_INT]),
         "pvglu_set_schedule": (C.c_int, [_VP, C.c_int, C.c_int]),
+        "pvglu_set_cache_policy": (C.c_int, [_VP, C.c_int]),
         "pvglu_get_schedule": (C.c_int, [_VP, _P_INT, _P_INT]),
         "pvglu_set_stream_groups": (C.c_int, [_VP, C.c_int]),
         "pvglu_get_stream_groups": (C.c_int, [_VP]),
@@ -198,6 +200,10 @@ class Plan:
 
     def set_stream_groups(self, groups):
         _check(lib().pvglu_set_stream_groups(self.handle, int(groups)))
+
+    def set_cache_policy(self, policy):
+        """CACHE_AUTO, CACHE_STREAMING or CACHE_DEFAULT (pvglu_set_cache_policy)."""
+        _check(lib().pvglu_set_cache_policy(self.handle, int(policy)))
 
     def get_stream_groups(self):
         return _check(lib().pvglu_get_stream_groups(self.handle))

@@ -391,14 +391,27 @@ template <int W>
 static void launch_one_level(pvglu_plan *P, const DevLevels &L, int first,
                              int nch, int op, const double *A, double *LU,
                              const double *b, double *x, int B,
-                             hipStream_t s) {
+                             hipStream_t s, bool stream) {
   const int tiles = (B + W - 1) / W;
   const long threads = (long)nch * tiles * W;
   const int blocks = (int)((threads + 255) / 256);
   if (blocks == 0)
     return;
   Launch<W>::level(pat_of(P->dev), lev_of(L), op, first, nch, tiles, A, LU,
-                   b, x, B, s);
+                   b, x, B, s, stream);
+}
+
+// Streaming cache policy (non-temporal hints on the value streams, width 64
+// only): on request, or by default when the batch's algorithmic bytes
+// exceed twice the 256 MB Infinity Cache (512 x 40x24x28: 0.286 -> 0.260 ms;
+// on a single instance or the 128-instance mixed batch the hints cost time).
+static bool use_streaming(const pvglu_plan *P, int W, int B) {
+  if (W != 64 || P->cache_policy == PVGLU_CACHE_DEFAULT)
+    return false;
+  if (P->cache_policy == PVGLU_CACHE_STREAMING)
+    return true;
+  const double bytes = (double)B * (24.0 * P->nnz + 16.0 * P->n);
+  return bytes > 512e6;
 }
 
 // Chains per component at or below which a level counts as narrow.
@@ -528,12 +541,13 @@ static int launch_steps_w(pvglu_plan *P, int mode, bool hybrid, int G,
       HIP_TRY(hipStreamWaitEvent(st, d->ev_skew[g - 1], 0));
     const int c0 = d->comp_bounds[gi][g], c1 = d->comp_bounds[gi][g + 1];
     bool skewed = false;
+    const bool stream = use_streaming(P, W, B);
     for (const Item &it : items) {
       if (!it.run) {
         const DevLevels &L = *sets[it.set];
         const int *r = &L.h_grp[gi][((size_t)g * L.nlev + it.lv) * 2];
         launch_one_level<W>(P, L, r[0], r[1] - r[0], it.op, A, LU, b, x, B,
-                            st);
+                            st, stream);
       } else {
         int slots = 1;
         while (slots < it.widest)
@@ -541,7 +555,7 @@ static int launch_steps_w(pvglu_plan *P, int mode, bool hybrid, int G,
         const int threads = std::max(64, slots * W);
         Launch<W>::hybrid(pat, lev_of(d->lu), lev_of(d->l), lev_of(d->u),
                           lev_of(d->lt), it.R, c0, c1 - c0, tiles, threads, A,
-                          LU, b, x, B, st);
+                          LU, b, x, B, st, stream);
       }
       if (g + 1 < G && !skewed) {
         // the next group starts once this group's first step is done
@@ -696,6 +710,17 @@ extern "C" int pvglu_set_stream_groups(pvglu_plan *P, int groups) {
   return PVGLU_OK;
 }
 
+extern "C" int pvglu_set_cache_policy(pvglu_plan *P, int policy) {
+  clear_error();
+  if (!P)
+    return fail(PVGLU_E_ARG, "set_cache_policy: null plan");
+  if (policy != PVGLU_CACHE_AUTO && policy != PVGLU_CACHE_STREAMING &&
+      policy != PVGLU_CACHE_DEFAULT)
+    return fail(PVGLU_E_ARG, "set_cache_policy: unknown policy %d", policy);
+  P->cache_policy = policy;
+  return PVGLU_OK;
+}
+
 extern "C" int pvglu_get_stream_groups(const pvglu_plan *P) {
   if (!P)
     return fail(PVGLU_E_ARG, "get_stream_groups: null plan");
@@ -764,7 +789,7 @@ __global__ void k_tune_values(int phase, const int *cp, const int *diag, int n,
 
 static std::mutex g_tune_mu;
 struct TuneChoice {
-  int variant, width, groups;
+  int variant, width, groups, policy;
 };
 static std::map<std::pair<uint64_t, int>, TuneChoice> g_tune_cache;
 
@@ -785,6 +810,7 @@ extern "C" int pvglu_autotune(pvglu_plan *P, int batch, int reps,
       P->sched_variant = it->second.variant;
       P->sched_width = it->second.width;
       P->sched_groups = it->second.groups;
+      P->cache_policy = it->second.policy;
       P->tuned_batch = batch;
       return PVGLU_OK;
     }
@@ -815,22 +841,29 @@ extern "C" int pvglu_autotune(pvglu_plan *P, int batch, int reps,
                        dim3(256), 0, s, phase, P->dev->cp, P->dev->diag_slot,
                        P->n, a, b, batch);
   const int save_v = P->sched_variant, save_w = P->sched_width,
-            save_g = P->sched_groups;
+            save_g = P->sched_groups, save_c = P->cache_policy;
   double best = 1e300;
-  int bv = PVGLU_SCHED_LEVEL, bw = 1, bg = 1;
+  int bv = PVGLU_SCHED_LEVEL, bw = 1, bg = 1, bc = PVGLU_CACHE_DEFAULT;
   const int widths[] = {1, 2, 4, 8, 16, 32, 64};
   for (int variant = PVGLU_SCHED_LEVEL; variant <= PVGLU_SCHED_HYBRID && !st;
        ++variant) {
     for (int w : widths) {
       if (w > 1 && w / 2 >= batch)
         continue; // wider tiles only idle lanes
-      for (int groups = 1; groups <= 4; groups *= 2) {
+      for (int cfg = 0; cfg < 6; ++cfg) {
+        // (groups, cache policy): streaming only where it can apply
+        const int groups = 1 << (cfg % 3);
+        const int policy = cfg < 3 ? PVGLU_CACHE_DEFAULT : PVGLU_CACHE_STREAMING;
         if (groups > 1 && (variant == PVGLU_SCHED_COMPONENT ||
                            !P->dev->side_ok || groups > P->ncomp))
+          continue;
+        if (policy == PVGLU_CACHE_STREAMING &&
+            (w != 64 || variant == PVGLU_SCHED_COMPONENT))
           continue;
         P->sched_variant = variant;
         P->sched_width = w;
         P->sched_groups = groups;
+        P->cache_policy = policy;
         st = pvglu_refactor_solve(P, a, lu, b, x, batch, s);
         if (st)
           break;
@@ -851,6 +884,7 @@ extern "C" int pvglu_autotune(pvglu_plan *P, int batch, int reps,
           bv = variant;
           bw = w;
           bg = groups;
+          bc = policy;
         }
       }
       if (st)
@@ -866,6 +900,7 @@ extern "C" int pvglu_autotune(pvglu_plan *P, int batch, int reps,
     P->sched_variant = save_v;
     P->sched_width = save_w;
     P->sched_groups = save_g;
+    P->cache_policy = save_c;
     return st;
   }
   // clear any status the stand-in values may have recorded
@@ -874,10 +909,11 @@ extern "C" int pvglu_autotune(pvglu_plan *P, int batch, int reps,
   P->sched_variant = bv;
   P->sched_width = bw;
   P->sched_groups = bg;
+  P->cache_policy = bc;
   P->tuned_batch = batch;
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
-    g_tune_cache[{P->hash, batch}] = {bv, bw, bg};
+    g_tune_cache[{P->hash, batch}] = {bv, bw, bg, bc};
   }
   if (best_ms)
     *best_ms = best;

@@ -97,6 +97,7 @@ struct pvglu_plan {
   int sched_variant = PVGLU_SCHED_AUTO;
   int sched_width = 0;
   int sched_groups = 1; // component groups on separate streams (1, 2, 4)
+  int cache_policy = 0;  // PVGLU_CACHE_AUTO / STREAMING / DEFAULT
   int tuned_batch = -1;
 
   pvglu::DevicePlan *dev = nullptr;

@@ -314,26 +314,53 @@ __device__ __forceinline__ void load_col(ColVals &c, int slot0, int col, int ne,
   c.bb = FUSE ? b[(size_t)col * B + inst] : 0.0;
 }
 
+// Streaming cache policy (STREAM = true; width 64 only, chosen per launch by
+// the host when the batch's values far exceed the 256 MB Infinity Cache).
+// A and b are read once per launch, and the strings' L entries and forward
+// results are not read again by the wide backward level before they would
+// be evicted, so those loads and stores are non-temporal; the backward
+// rows' loads are too.  The U entries and pivots keep the default policy:
+// the backward level re-reads them, partly from the Infinity Cache.  Measured
+// on 512 x 40x24x28 against the default policy: LU level 0 163.7 -> 145.6
+// us, the narrow run 39.2 -> 35.1 us, U level 2 82.6 -> 76.5 us
+// (non-temporal U stores slowed level 0 by 13 us for 3 us in level 2).  On
+// small working sets (one instance, the 128-instance mixed batch) the same
+// hints lose the cache reuse and cost time, so there they are off.
+template <bool STREAM>
+__device__ __forceinline__ double ldv(const double *p) {
+  if constexpr (STREAM)
+    return __builtin_nontemporal_load(p);
+  else
+    return *p;
+}
+template <bool STREAM>
+__device__ __forceinline__ void stv(double *p, double v) {
+  if constexpr (STREAM)
+    __builtin_nontemporal_store(v, p);
+  else
+    *p = v;
+}
+
 // A segment whose operands are all forwarded from the previous column runs
 // as a three-stage pipeline with a fixed sequence of memory operations per
 // column (the loads of column i+2 are issued before column i computes, stores
 // are unconditional in NE, the loop is unrolled three times so no register
 // copies of in-flight loads are needed): the compiler's counter-based waits
 // then wait for the column being computed only, not for the prefetches.
-template <int NE, bool FUSE>
+template <int NE, bool FUSE, bool STREAM = false>
 __device__ __forceinline__ void pure_load_col(ColVals &c, int slot0, int col,
                                               const double *A,
                                               const double *b, size_t B,
                                               int inst) {
-  c.a0 = A[(size_t)slot0 * B + inst];
+  c.a0 = ldv<STREAM>(&A[(size_t)slot0 * B + inst]);
   if constexpr (NE > 1)
-    c.a1 = A[(size_t)(slot0 + 1) * B + inst];
+    c.a1 = ldv<STREAM>(&A[(size_t)(slot0 + 1) * B + inst]);
   if constexpr (NE > 2)
-    c.a2 = A[(size_t)(slot0 + 2) * B + inst];
+    c.a2 = ldv<STREAM>(&A[(size_t)(slot0 + 2) * B + inst]);
   if constexpr (NE > 3)
-    c.a3 = A[(size_t)(slot0 + 3) * B + inst];
+    c.a3 = ldv<STREAM>(&A[(size_t)(slot0 + 3) * B + inst]);
   if constexpr (FUSE)
-    c.bb = b[(size_t)col * B + inst];
+    c.bb = ldv<STREAM>(&b[(size_t)col * B + inst]);
 }
 
 template <int NE, bool FUSE>
@@ -418,7 +445,7 @@ __device__ __forceinline__ void pipeline(int len, LD ld, RUN run) {
 constexpr int kLuDepth = 2, kRowDepth = 2;
 constexpr int kNarrowLuDepth = 8, kNarrowRowDepth = 8;
 
-template <int NE, bool FUSE, int D = kLuDepth>
+template <int NE, bool FUSE, int D = kLuDepth, bool STREAM = false>
 __device__ __forceinline__ void pure_lu_segment(
     const int (&f)[16], int len, int dslot, int dcol, const double *A,
     double *LU, const double *b, double *x, size_t B, int inst, double &p0,
@@ -427,8 +454,8 @@ __device__ __forceinline__ void pure_lu_segment(
       len,
       [&](ColVals &c, int i) {
         i = min(i, len - 1);
-        pure_load_col<NE, FUSE>(c, f[0] + i * dslot, f[1] + i * dcol, A, b, B,
-                                inst);
+        pure_load_col<NE, FUSE, STREAM>(c, f[0] + i * dslot, f[1] + i * dcol,
+                                        A, b, B, inst);
       },
       [&](const ColVals &c, int i) {
         pure_lu_col<NE, FUSE>(c, f[0] + i * dslot, f[1] + i * dcol, f, LU, x,
@@ -445,7 +472,7 @@ __device__ __forceinline__ void pure_lu_segment(
 // the forwarded entry at compile time (the string columns), and small
 // pivots are looked for once per segment (running minimum, then a rescan
 // of the stored pivots in the rare case).
-template <bool FUSE, int D, int KC = -1>
+template <bool FUSE, int D, int KC = -1, bool STREAM = false>
 __device__ __forceinline__ void tri_lu_segment(
     int slot0, int col0, int len, int dslot, int dcol, int k, const double *A,
     double *LU, const double *b, double *x, size_t B, int inst, double &p0,
@@ -458,8 +485,8 @@ __device__ __forceinline__ void tri_lu_segment(
       len,
       [&](ColVals &c, int i) {
         i = min(i, len - 1);
-        pure_load_col<3, FUSE>(c, slot0 + i * dslot, col0 + i * dcol, A, b, B,
-                               inst);
+        pure_load_col<3, FUSE, STREAM>(c, slot0 + i * dslot, col0 + i * dcol,
+                                       A, b, B, inst);
       },
       [&](const ColVals &c, int i) {
         const size_t s = (size_t)(slot0 + i * dslot) * B + inst;
@@ -469,10 +496,10 @@ __device__ __forceinline__ void tri_lu_segment(
         const double lo = c.a2 / piv;
         LU[s] = u;
         LU[s + B] = piv;
-        LU[s + 2 * B] = lo;
+        stv<STREAM>(&LU[s + 2 * B], lo);
         if constexpr (FUSE) {
           const double y = c.bb - pl * yprev;
-          x[(size_t)(col0 + i * dcol) * B + inst] = y;
+          stv<STREAM>(&x[(size_t)(col0 + i * dcol) * B + inst], y);
           yprev = y;
         }
         v0 = u;
@@ -610,7 +637,7 @@ struct RowVals {
 // Row sweeps share one register program: x_i = (src_i - sum_j T(i,j) x_j)
 // [/ T(i,i) when DIV], T = U (backward, src = x in place, DIV) or unit L
 // (forward, src = b).  Records hold slots relative to the row's diagonal slot.
-template <int NT, bool DIV, int D = kRowDepth>
+template <int NT, bool DIV, int D = kRowDepth, bool STREAM = false>
 __device__ __forceinline__ void pure_row_segment(const int (&f)[8], int len,
                                                  int drow, int ddiag,
                                                  const double *LU,
@@ -622,11 +649,11 @@ __device__ __forceinline__ void pure_row_segment(const int (&f)[8], int len,
       [&](RowVals &r, int i) {
         i = min(i, len - 1);
         const int row = f[0] + i * drow, dslot = f[1] + i * ddiag;
-        r.y = src[(size_t)row * B + inst];
+        r.y = ldv<STREAM>(&src[(size_t)row * B + inst]);
         if constexpr (DIV)
-          r.dv = LU[(size_t)dslot * B + inst];
+          r.dv = ldv<STREAM>(&LU[(size_t)dslot * B + inst]);
         if constexpr (NT > 0)
-          r.u0 = LU[(size_t)(dslot + f[4]) * B + inst];
+          r.u0 = ldv<STREAM>(&LU[(size_t)(dslot + f[4]) * B + inst]);
       },
       [&](const RowVals &r, int i) {
         double s = r.y;
@@ -640,7 +667,7 @@ __device__ __forceinline__ void pure_row_segment(const int (&f)[8], int len,
 
 // p0..p3 / yprev: the previous column's window (zero at a chain's start),
 // updated to this run's last column.
-template <int W, bool FUSE, int D = kLuDepth>
+template <int W, bool FUSE, int D = kLuDepth, bool STREAM = false>
 __device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
                                               const double *A, double *LU,
                                               const double *b, double *x,
@@ -661,12 +688,14 @@ __device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
     if (sflag & (FUSE ? 8 : 4)) {
       int badcol = -1;
       if (((sflag >> 4) & 3) == 2)
-        tri_lu_segment<FUSE, D, 2>(f[0], f[1], len, dslot, dcol, 2, A, LU, b,
-                                   x, B, inst, p0, p1, p2, p3, yprev, badcol);
+        tri_lu_segment<FUSE, D, 2, STREAM>(f[0], f[1], len, dslot, dcol, 2, A,
+                                           LU, b, x, B, inst, p0, p1, p2, p3,
+                                           yprev, badcol);
       else
-        tri_lu_segment<FUSE, D>(f[0], f[1], len, dslot, dcol,
-                                (sflag >> 4) & 3, A, LU, b, x, B, inst, p0, p1,
-                                p2, p3, yprev, badcol);
+        tri_lu_segment<FUSE, D, -1, STREAM>(f[0], f[1], len, dslot, dcol,
+                                            (sflag >> 4) & 3, A, LU, b, x, B,
+                                            inst, p0, p1, p2, p3, yprev,
+                                            badcol);
       if (badcol >= 0)
         record_status(P.status, PVGLU_E_SINGULAR, badcol, inst);
       t += len;
@@ -676,19 +705,19 @@ __device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
       int badcol = -1;
       switch (ne) {
       case 1:
-        pure_lu_segment<1, FUSE, D>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
+        pure_lu_segment<1, FUSE, D, STREAM>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
                            p2, p3, yprev, badcol);
         break;
       case 2:
-        pure_lu_segment<2, FUSE, D>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
+        pure_lu_segment<2, FUSE, D, STREAM>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
                            p2, p3, yprev, badcol);
         break;
       case 3:
-        pure_lu_segment<3, FUSE, D>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
+        pure_lu_segment<3, FUSE, D, STREAM>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
                            p2, p3, yprev, badcol);
         break;
       default:
-        pure_lu_segment<4, FUSE, D>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
+        pure_lu_segment<4, FUSE, D, STREAM>(f, len, dslot, dcol, A, LU, b, x, B, inst, p0, p1,
                            p2, p3, yprev, badcol);
         break;
       }
@@ -775,7 +804,7 @@ __device__ __forceinline__ void load_row(RowVals &v, int row, int dslot, int nt,
   v.u1 = nt > 1 ? LU[(size_t)(dslot + rel1) * B + inst] : 0.0;
 }
 
-template <int W, bool DIV, int D = kRowDepth>
+template <int W, bool DIV, int D = kRowDepth, bool STREAM = false>
 __device__ __forceinline__ void fast_row_chain(const int *rec, const int *seg,
                                                int t0, int t1,
                                                const double *LU,
@@ -792,10 +821,10 @@ __device__ __forceinline__ void fast_row_chain(const int *rec, const int *seg,
     const int nt = f[2];
     if (uniform<W>(sg[3])) {
       if (nt == 0)
-        pure_row_segment<0, DIV, D>(f, len, drow, ddiag, LU, src, x, B, inst,
+        pure_row_segment<0, DIV, D, STREAM>(f, len, drow, ddiag, LU, src, x, B, inst,
                                  xprev);
       else
-        pure_row_segment<1, DIV, D>(f, len, drow, ddiag, LU, src, x, B, inst,
+        pure_row_segment<1, DIV, D, STREAM>(f, len, drow, ddiag, LU, src, x, B, inst,
                                  xprev);
       t += len;
       continue;
@@ -828,7 +857,7 @@ __device__ __forceinline__ void fast_row_chain(const int *rec, const int *seg,
 
 // NAR = 1: the narrow-run kernel's variant (deep register pipelines, star
 // columns loaded in one round trip).
-template <int W, int OP, int NAR = 0>
+template <int W, int OP, int NAR = 0, bool STREAM = false>
 __device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
                                           const double *A, double *LU,
                                           const double *b, double *x, size_t B,
@@ -841,8 +870,8 @@ __device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
     constexpr bool FUSE = OP == OP_REFACTOR_FUSED;
     double p0 = 0, p1 = 0, p2 = 0, p3 = 0, yprev = 0;
     if (uniform<W>(P.lu_fast[chain])) {
-      fast_lu_chain<W, FUSE, DL>(P, t0, t1, A, LU, b, x, B, inst, p0, p1, p2,
-                                 p3, yprev);
+      fast_lu_chain<W, FUSE, DL, STREAM>(P, t0, t1, A, LU, b, x, B, inst, p0,
+                                         p1, p2, p3, yprev);
       return;
     }
     // mixed chain: register runs between generic columns; a run's first
@@ -850,8 +879,8 @@ __device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
     for (int t = t0; t < t1;) {
       const int e = uniform<W>(P.lu_run[t]);
       if (e > t) {
-        fast_lu_chain<W, FUSE, DL>(P, t, e, A, LU, b, x, B, inst, p0, p1, p2,
-                                   p3, yprev);
+        fast_lu_chain<W, FUSE, DL, STREAM>(P, t, e, A, LU, b, x, B, inst, p0,
+                                           p1, p2, p3, yprev);
         t = e;
       } else {
         const int c = uniform<W>(L.cols[t]);
@@ -888,13 +917,15 @@ __device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
     // source of the right-hand side: forward b, U^T g; in place otherwise
     const double *src = (OP == OP_LSOLVE || OP == OP_UT) ? b : x;
     if (uniform<W>(whole[chain])) {
-      fast_row_chain<W, DIV, DR>(rec, seg, t0, t1, LU, src, x, B, inst);
+      fast_row_chain<W, DIV, DR, STREAM>(rec, seg, t0, t1, LU, src, x, B,
+                                         inst);
       return;
     }
     for (int t = t0; t < t1;) {
       const int e = uniform<W>(run[t]);
       if (e > t) {
-        fast_row_chain<W, DIV, DR>(rec, seg, t, e, LU, src, x, B, inst);
+        fast_row_chain<W, DIV, DR, STREAM>(rec, seg, t, e, LU, src, x, B,
+                                           inst);
         t = e;
       } else {
         const int c = uniform<W>(L.cols[t]);
@@ -931,7 +962,7 @@ __device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
 // before, and in reverse order its first blocks read what was written last,
 // which the 256 MB Infinity Cache still holds (measured: 10 us less per
 // 512-instance refactor+solve of the 40x24x28 plant).
-template <int W, int OP>
+template <int W, int OP, bool STREAM = false>
 __global__ void __launch_bounds__(256)
     k_level(Pat P, Lev L, int first_chain, int nchains, int tiles,
             const double *A, double *LU, const double *b, double *x, int B) {
@@ -949,14 +980,14 @@ __global__ void __launch_bounds__(256)
   const int inst = tile * W + lane;
   if (inst >= B)
     return;
-  run_chain<W, OP>(P, L, chain, A, LU, b, x, (size_t)B, inst);
+  run_chain<W, OP, 0, STREAM>(P, L, chain, A, LU, b, x, (size_t)B, inst);
 }
 
 // COMPONENT schedule: one launch for the whole refactor + solve.  Block ->
 // (component, instance tile); each level of the component is split over the
 // block's chain slots; barriers separate levels.  Every thread reaches every
 // barrier (loop bounds are block-uniform; inactive lanes only skip work).
-template <int W, int NAR>
+template <int W, int NAR, bool STREAM = false>
 __device__ __forceinline__ void component_levels(const Pat &P, const Lev &L,
                                                  int comp, int slot, int slots,
                                                  bool active, const double *A,
@@ -974,17 +1005,23 @@ __device__ __forceinline__ void component_levels(const Pat &P, const Lev &L,
       for (int ch = c0 + slot; ch < c1; ch += slots) {
         const int chain = uniform<W>(ch);
         if (op == OP_REFACTOR_FUSED)
-          run_chain<W, OP_REFACTOR_FUSED, NAR>(P, L, chain, A, LU, b, x, B, inst);
+          run_chain<W, OP_REFACTOR_FUSED, NAR, STREAM>(P, L, chain, A, LU, b, x, B,
+                                              inst);
         else if (op == OP_REFACTOR)
-          run_chain<W, OP_REFACTOR, NAR>(P, L, chain, A, LU, b, x, B, inst);
+          run_chain<W, OP_REFACTOR, NAR, STREAM>(P, L, chain, A, LU, b, x, B,
+                                              inst);
         else if (op == OP_LSOLVE)
-          run_chain<W, OP_LSOLVE, NAR>(P, L, chain, A, LU, b, x, B, inst);
+          run_chain<W, OP_LSOLVE, NAR, STREAM>(P, L, chain, A, LU, b, x, B,
+                                              inst);
         else if (op == OP_USOLVE)
-          run_chain<W, OP_USOLVE, NAR>(P, L, chain, A, LU, b, x, B, inst);
+          run_chain<W, OP_USOLVE, NAR, STREAM>(P, L, chain, A, LU, b, x, B,
+                                              inst);
         else if (op == OP_UT)
-          run_chain<W, OP_UT, NAR>(P, L, chain, A, LU, b, x, B, inst);
+          run_chain<W, OP_UT, NAR, STREAM>(P, L, chain, A, LU, b, x, B,
+                                              inst);
         else
-          run_chain<W, OP_LT, NAR>(P, L, chain, A, LU, b, x, B, inst);
+          run_chain<W, OP_LT, NAR, STREAM>(P, L, chain, A, LU, b, x, B,
+                                              inst);
       }
     __syncthreads();
   }
@@ -1042,7 +1079,7 @@ __global__ void __launch_bounds__(1024)
 // the chains use the deep register pipelines and star columns take the
 // one-round-trip path.  At most 4 chain slots of W <= 64 lanes (a narrow
 // level has at most 4 chains per component), so at most 256 threads.
-template <int W>
+template <int W, bool STREAM = false>
 __global__ void __launch_bounds__(256)
     k_hybrid(Pat P, Lev Llu, Lev Ll, Lev Lu, Lev Llt, LvRange R, int comp0,
              int tiles, const double *A, double *LU, const double *b,
@@ -1061,8 +1098,8 @@ __global__ void __launch_bounds__(256)
     if (lo >= hi)
       continue;
     const Lev Lk = k == 0 ? Llu : k == 1 ? Ll : k == 2 ? Lu : Llt;
-    component_levels<W, 1>(P, Lk, comp, slot, slots, active, A, LU, b, x,
-                           (size_t)B, inst, op, lo, hi);
+    component_levels<W, 1, STREAM>(P, Lk, comp, slot, slots, active, A, LU,
+                                   b, x, (size_t)B, inst, op, lo, hi);
   }
 }
 
@@ -1136,14 +1173,23 @@ __global__ void __launch_bounds__(1024) k_multi(MultiArgs m, int mode) {
 template <int W>
 void Launch<W>::level(const Pat &pat, const Lev &lev, int op, int first,
                       int nch, int tiles, const double *A, double *LU,
-                      const double *b, double *x, int B, hipStream_t s) {
+                      const double *b, double *x, int B, hipStream_t s,
+                      bool stream) {
   const long threads = (long)nch * tiles * W;
   const int blocks = (int)((threads + 255) / 256);
   if (blocks == 0)
     return;
+  constexpr bool kStreamable = W == 64; // streaming policy: width 64 only
 #define PVGLU_LVL(OPC)                                                         \
-  hipLaunchKernelGGL((k_level<W, OPC>), dim3(blocks), dim3(256), 0, s, pat,    \
-                     lev, first, nch, tiles, A, LU, b, x, B)
+  do {                                                                         \
+    if (kStreamable && stream)                                                 \
+      hipLaunchKernelGGL((k_level<W, OPC, kStreamable>), dim3(blocks),         \
+                         dim3(256), 0, s, pat, lev, first, nch, tiles, A, LU,  \
+                         b, x, B);                                             \
+    else                                                                       \
+      hipLaunchKernelGGL((k_level<W, OPC>), dim3(blocks), dim3(256), 0, s,     \
+                         pat, lev, first, nch, tiles, A, LU, b, x, B);         \
+  } while (0)
   switch (op) {
   case OP_REFACTOR: PVGLU_LVL(OP_REFACTOR); break;
   case OP_REFACTOR_FUSED: PVGLU_LVL(OP_REFACTOR_FUSED); break;
@@ -1160,8 +1206,15 @@ void Launch<W>::hybrid(const Pat &pat, const Lev &lu, const Lev &l,
                        const Lev &u, const Lev &lt, const LvRange &R,
                        int comp0, int ncomp, int tiles, int threads,
                        const double *A, double *LU, const double *b, double *x,
-                       int B, hipStream_t s) {
-  if (ncomp > 0)
+                       int B, hipStream_t s, bool stream) {
+  if (ncomp <= 0)
+    return;
+  constexpr bool kStreamable = W == 64;
+  if (kStreamable && stream)
+    hipLaunchKernelGGL((k_hybrid<W, kStreamable>), dim3(ncomp * tiles),
+                       dim3(threads), 0, s, pat, lu, l, u, lt, R, comp0,
+                       tiles, A, LU, b, x, B);
+  else
     hipLaunchKernelGGL((k_hybrid<W>), dim3(ncomp * tiles), dim3(threads), 0, s,
                        pat, lu, l, u, lt, R, comp0, tiles, A, LU, b, x, B);
 }

@@ -131,15 +131,17 @@ struct MultiRun {
 template <int W> struct Launch {
   // one level (chains [first, first + nch) of a level set) with operation op;
   // grid: ceil(nch * tiles * W / 256) blocks of 256 threads
+  // stream: non-temporal hints on the streaming accesses (W == 64 only)
   static void level(const Pat &pat, const Lev &lev, int op, int first,
                     int nch, int tiles, const double *A, double *LU,
-                    const double *b, double *x, int B, hipStream_t s);
+                    const double *b, double *x, int B, hipStream_t s,
+                    bool stream = false);
   // a run of narrow levels, block -> (component comp0 + k, tile)
   static void hybrid(const Pat &pat, const Lev &lu, const Lev &l,
                      const Lev &u, const Lev &lt, const LvRange &R, int comp0,
                      int ncomp, int tiles, int threads, const double *A,
                      double *LU, const double *b, double *x, int B,
-                     hipStream_t s);
+                     hipStream_t s, bool stream = false);
   // the whole refactor / solve, block -> (component, tile)
   static void component(const Pat &pat, const Lev &lu, const Lev &l,
                         const Lev &u, const Lev &lt, int mode, bool fuse,

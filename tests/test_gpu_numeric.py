@@ -303,6 +303,54 @@ def test_singular_pivot_in_a_string_segment(dev, sched):
     assert plan.status() == (pv.E_SINGULAR, 3, 37)
 
 
+@pytest.mark.parametrize("sched", [(pv.SCHED_HYBRID, 64), (pv.SCHED_LEVEL, 64)])
+def test_streaming_cache_policy_bitexact(dev, sched):
+    """The width-64 kernels with the streaming cache policy (non-temporal
+    hints, chosen by default only for batches far beyond the Infinity Cache)
+    forced on a 130-instance batch of the 40x24x28 plant: refactor, both
+    sweeps and the transposed sweeps bit-exact vs the oracle and equal to
+    the default policy's results."""
+    I, S, M = 40, 24, 28
+    rng = np.random.default_rng(31)
+    n, rows = plant_pattern(I, S, M)
+    rp0, ci0 = csr_from_sets(rows)
+    P = O.amd_order(n, rp0, ci0) if O.ref() is not None else np.arange(n, dtype=np.int32)
+    rp, ci, _ = O.permute_sym(n, rp0, ci0, np.zeros(len(ci0)), P)
+    s = O.csr_csc(n, rp, ci)
+    B = 130
+    _, _, _, V, bv = plant_values(rng, I, S, M, B)
+    order = O.permute_sym(n, rp0, ci0, np.arange(len(ci0), dtype=float), P)[2].astype(np.int64)
+    A = np.ascontiguousarray(V[order][s.csc2csr])
+    bb = np.zeros((n, B))
+    bb[O.invert_permutation(P)] = bv
+    out = {}
+    for policy in (pv.CACHE_STREAMING, pv.CACHE_DEFAULT):
+        plan = pv.Plan(n, rp, ci)
+        plan.set_schedule(*sched)
+        plan.set_cache_policy(policy)
+        dA, db = _to_dev(A, dev), _to_dev(bb, dev)
+        dlu, dx, dy = torch.empty_like(dA), torch.empty_like(db), torch.empty_like(db)
+        plan.status_reset()
+        plan.refactor_solve(dA, dlu, db, dx, B)
+        plan.solve_transpose(dlu, db, dy, B)
+        dlu2, dx2 = torch.empty_like(dA), torch.empty_like(db)
+        plan.refactor(dA, dlu2, B)
+        plan.solve(dlu2, db, dx2, B)
+        torch.cuda.synchronize()
+        assert plan.status()[0] == 0
+        assert torch.equal(dlu2, dlu) and torch.equal(dx2, dx)
+        out[policy] = (dlu.cpu().numpy(), dx.cpu().numpy(), dy.cpu().numpy())
+    lu, x, y = out[pv.CACHE_STREAMING]
+    for a, b_ in zip(out[pv.CACHE_STREAMING], out[pv.CACHE_DEFAULT]):
+        assert np.array_equal(a, b_)
+    for k in (0, 65, B - 1):
+        wlu, wx = _oracle_lu_solve(s, A[:, k].copy(), bb[:, k].copy())
+        assert np.array_equal(lu[:, k], wlu) and np.array_equal(x[:, k], wx)
+        g = bb[:, k].copy()
+        O.solve_transpose_csc(s, wlu.copy(), g)
+        assert np.array_equal(y[:, k], g)
+
+
 def test_singular_pivot_is_reported(dev):
     rng, n, rp, ci = _random_case(3, n=40)
     plan = pv.Plan(n, rp, ci)
