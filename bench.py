@@ -10,8 +10,9 @@ instances sharing one pattern.  The systems are the plant's companion-model
 Jacobians stamped on the GPU at the converged Newton state of each scenario
 (batched device Newton to 1e-10 V), so values are device-resident when the
 timed region starts.  Beside it on the same line:
-  * plain: the single-instance GPU path (batch 1, one kernel per level, the
-    reference's launch structure) on the same plant;
+  * plain: the single-instance GPU path on the same plant (batch 1, the
+    library's default schedule), with the reference's launch structure (one
+    kernel per level, one lane per chain) timed beside it;
   * dropin: the reference call sequence end to end (host assembly, copies,
     GPU refactor/solve/update) per Newton iteration;
   * roofline of the refactor+solve launch and the reference CPU GLU baseline.
@@ -325,34 +326,47 @@ def main():
     if not args.no_plain:
         p1 = pv.PV(args.inverters, args.strings, args.modules, seed=args.seed)
         plan1 = p1.plan()
-        plan1.set_schedule(pv.SCHED_LEVEL, 1)
         A1 = A[:, :1].contiguous()
         b1 = b[:, :1].contiguous()
         LU1 = torch.empty_like(A1)
         X1 = torch.empty_like(b1)
-        for _ in range(args.warmup):
-            plan1.refactor_solve(A1, LU1, b1, X1, 1, stream)
-        torch.cuda.synchronize()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        reps = max(args.steps, 20)
-        e0.record(stream)
-        for _ in range(reps):
-            plan1.refactor_solve(A1, LU1, b1, X1, 1, stream)
-        e1.record(stream)
-        torch.cuda.synchronize()
-        plain_ms = e0.elapsed_time(e1) / reps
+
+        def time_single():
+            for _ in range(args.warmup):
+                plan1.refactor_solve(A1, LU1, b1, X1, 1, stream)
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            reps = max(args.steps, 20)
+            e0.record(stream)
+            for _ in range(reps):
+                plan1.refactor_solve(A1, LU1, b1, X1, 1, stream)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / reps
+
+        # the library's single-instance path (AUTO: one COMPONENT launch) and
+        # the reference's launch structure (one kernel per level, one lane
+        # per chain) beside it
+        plain_ms = time_single()
+        sv, sw = plan1.get_schedule()
+        plan1.set_schedule(pv.SCHED_LEVEL, 1)
+        level_ms = time_single()
+        plan1.set_schedule(sv, sw)
         plain = 1e3 / plain_ms
         out["plain"] = {"value": plain, "unit": "refactor+solve/s", "ms_per_solve": plain_ms,
-                        "schedule": "one kernel per level, 1 lane per chain"}
+                        "schedule": "library default at batch 1 (COMPONENT, one launch)",
+                        "reference_structure_ms_per_solve": level_ms,
+                        "reference_structure": "one kernel per level, 1 lane per chain"}
         out["batched_over_plain"] = value / plain
         # drop-in: the reference call sequence per Newton iteration
         xd, vd, st = p1.newton_dropin(G[:, 0], v=v_n[:, 0], fixed_iters=5)
         out["dropin"] = {"ms_per_newton_iteration": st[3] / st[0],
                          "host_assembly_ms": st[1] / st[0], "device_ms": st[2] / st[0],
                          "note": "host assembly + H2D/D2H + GPU refactor/solve/update"}
-        print(f"[bench] plain: {plain_ms:.4f} ms/solve ({plain:.4g}/s); batched/plain "
-              f"{value / plain:.1f}x; drop-in {st[3] / st[0]:.2f} ms/iteration", flush=True)
+        print(f"[bench] plain: {plain_ms:.4f} ms/solve ({plain:.4g}/s; reference launch "
+              f"structure {level_ms:.4f} ms); batched/plain {value / plain:.1f}x; drop-in "
+              f"{st[3] / st[0]:.2f} ms/iteration", flush=True)
 
     if not args.no_cpu:
         cb = cpu_baseline(plan_arrays, A_host, b_host)

@@ -189,7 +189,8 @@ int pvglu_get_schedule(const pvglu_plan *plan, int *variant, int *width);
  * need G = 1.  No counterpart in the reference. */
 int pvglu_set_stream_groups(pvglu_plan *plan, int groups);
 
-/* Cache policy of the width-64 LEVEL and HYBRID kernels' value streams.
+/* Cache policy of the width-32 and width-64 LEVEL and HYBRID kernels' value
+ * streams.
  * STREAMING marks the loads of A, b and the substitution operands and the
  * stores of L entries and forward results non-temporal (they are not read
  * again before eviction); DEFAULT leaves every access to the cache; AUTO

@@ -401,12 +401,13 @@ static void launch_one_level(pvglu_plan *P, const DevLevels &L, int first,
                    b, x, B, s, stream);
 }
 
-// Streaming cache policy (non-temporal hints on the value streams, width 64
-// only): on request, or by default when the batch's algorithmic bytes
+// Streaming cache policy (non-temporal hints on the value streams, widths 32
+// and 64): on request, or by default when the batch's algorithmic bytes
 // exceed twice the 256 MB Infinity Cache (512 x 40x24x28: 0.286 -> 0.260 ms;
-// on a single instance or the 128-instance mixed batch the hints cost time).
+// 32 x 400x24x28 at width 32: 0.197 -> 0.184 ms; on a single instance or the
+// 128-instance mixed batch the hints cost time).
 static bool use_streaming(const pvglu_plan *P, int W, int B) {
-  if (W != 64 || P->cache_policy == PVGLU_CACHE_DEFAULT)
+  if (W < 32 || P->cache_policy == PVGLU_CACHE_DEFAULT)
     return false;
   if (P->cache_policy == PVGLU_CACHE_STREAMING)
     return true;
@@ -858,7 +859,7 @@ extern "C" int pvglu_autotune(pvglu_plan *P, int batch, int reps,
                            !P->dev->side_ok || groups > P->ncomp))
           continue;
         if (policy == PVGLU_CACHE_STREAMING &&
-            (w != 64 || variant == PVGLU_SCHED_COMPONENT))
+            (w < 32 || variant == PVGLU_SCHED_COMPONENT))
           continue;
         P->sched_variant = variant;
         P->sched_width = w;

@@ -314,7 +314,7 @@ __device__ __forceinline__ void load_col(ColVals &c, int slot0, int col, int ne,
   c.bb = FUSE ? b[(size_t)col * B + inst] : 0.0;
 }
 
-// Streaming cache policy (STREAM = true; width 64 only, chosen per launch by
+// Streaming cache policy (STREAM = true; widths 32 and 64, chosen per launch by
 // the host when the batch's values far exceed the 256 MB Infinity Cache).
 // A and b are read once per launch, and the strings' L entries and forward
 // results are not read again by the wide backward level before they would
@@ -1179,7 +1179,7 @@ void Launch<W>::level(const Pat &pat, const Lev &lev, int op, int first,
   const int blocks = (int)((threads + 255) / 256);
   if (blocks == 0)
     return;
-  constexpr bool kStreamable = W == 64; // streaming policy: width 64 only
+  constexpr bool kStreamable = W >= 32; // streaming policy: widths 32, 64
 #define PVGLU_LVL(OPC)                                                         \
   do {                                                                         \
     if (kStreamable && stream)                                                 \
@@ -1209,7 +1209,7 @@ void Launch<W>::hybrid(const Pat &pat, const Lev &lu, const Lev &l,
                        int B, hipStream_t s, bool stream) {
   if (ncomp <= 0)
     return;
-  constexpr bool kStreamable = W == 64;
+  constexpr bool kStreamable = W >= 32;
   if (kStreamable && stream)
     hipLaunchKernelGGL((k_hybrid<W, kStreamable>), dim3(ncomp * tiles),
                        dim3(threads), 0, s, pat, lu, l, u, lt, R, comp0,

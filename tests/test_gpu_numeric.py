@@ -303,9 +303,10 @@ def test_singular_pivot_in_a_string_segment(dev, sched):
     assert plan.status() == (pv.E_SINGULAR, 3, 37)
 
 
-@pytest.mark.parametrize("sched", [(pv.SCHED_HYBRID, 64), (pv.SCHED_LEVEL, 64)])
+@pytest.mark.parametrize("sched", [(pv.SCHED_HYBRID, 64), (pv.SCHED_LEVEL, 64),
+                                   (pv.SCHED_HYBRID, 32)])
 def test_streaming_cache_policy_bitexact(dev, sched):
-    """The width-64 kernels with the streaming cache policy (non-temporal
+    """The width-32/64 kernels with the streaming cache policy (non-temporal
     hints, chosen by default only for batches far beyond the Infinity Cache)
     forced on a 130-instance batch of the 40x24x28 plant: refactor, both
     sweeps and the transposed sweeps bit-exact vs the oracle and equal to
