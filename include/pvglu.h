@@ -205,10 +205,12 @@ enum {
 int pvglu_set_cache_policy(pvglu_plan *plan, int policy);
 int pvglu_get_stream_groups(const pvglu_plan *plan);
 
-/* Times the schedule candidates (variant x width x component groups) for this
- * batch size on scratch buffers (allocates and synchronizes: call it outside
- * any capture) and keeps the fastest.  Results are cached per (pattern hash,
- * batch). */
+/* Times the schedule candidates (variant x width x component groups x cache
+ * policy) for this batch size on scratch buffers (allocates and
+ * synchronizes: call it outside any capture) and keeps the fastest.  Results
+ * are cached per (pattern hash, batch): with best_ms NULL a cached choice is
+ * taken without timing; with best_ms set the candidates are timed again and
+ * *best_ms receives the fastest median. */
 int pvglu_autotune(pvglu_plan *plan, int batch, int reps, double *best_ms);
 
 /* ------------------------------------------------------------- numeric */

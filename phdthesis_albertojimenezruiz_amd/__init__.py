@@ -208,7 +208,13 @@ class Plan:
     def get_stream_groups(self):
         return _check(lib().pvglu_get_stream_groups(self.handle))
 
-    def autotune(self, batch, reps=5):
+    def autotune(self, batch, reps=5, cached=False):
+        """Times the candidate schedules (pvglu_autotune) and returns the best
+        time in ms; cached=True takes a choice already made for this pattern
+        and batch without timing (returns 0.0 then)."""
+        if cached:
+            _check(lib().pvglu_autotune(self.handle, int(batch), int(reps), None))
+            return 0.0
         best = C.c_double()
         _check(lib().pvglu_autotune(self.handle, int(batch), int(reps), C.byref(best)))
         return best.value

@@ -352,6 +352,39 @@ def test_streaming_cache_policy_bitexact(dev, sched):
         assert np.array_equal(y[:, k], g)
 
 
+def test_autotune_choice_is_bitexact_and_cached(dev):
+    """pvglu_autotune times every (schedule, width, groups, cache policy) on
+    scratch values, keeps the fastest, and caches it per (pattern, batch);
+    the tuned launch stays bit-exact vs the oracle."""
+    I, S, M = 4, 10, 20
+    rng = np.random.default_rng(5)
+    n, rows = plant_pattern(I, S, M)
+    rp, ci = csr_from_sets(rows)
+    s = O.csr_csc(n, rp, ci)
+    B = 70
+    _, _, _, V, bv = plant_values(rng, I, S, M, B)
+    A = np.ascontiguousarray(V[s.csc2csr])
+    plan = pv.Plan(n, rp, ci)
+    best = plan.autotune(B, reps=2)
+    assert 0.0 < best < 100.0
+    choice = plan.get_schedule()
+    assert choice[0] in (pv.SCHED_LEVEL, pv.SCHED_COMPONENT, pv.SCHED_HYBRID)
+    assert choice[1] in (1, 2, 4, 8, 16, 32, 64)
+    plan2 = pv.Plan(n, rp, ci)                         # same pattern: cached
+    plan2.autotune(B, reps=2, cached=True)
+    assert plan2.get_schedule() == choice
+    dA, db = _to_dev(A, dev), _to_dev(bv, dev)
+    dlu, dx = torch.empty_like(dA), torch.empty_like(db)
+    plan.status_reset()
+    plan.refactor_solve(dA, dlu, db, dx, B)
+    torch.cuda.synchronize()
+    assert plan.status()[0] == 0
+    lu, x = dlu.cpu().numpy(), dx.cpu().numpy()
+    for k in (0, 33, B - 1):
+        wlu, wx = _oracle_lu_solve(s, A[:, k].copy(), bv[:, k].copy())
+        assert np.array_equal(lu[:, k], wlu) and np.array_equal(x[:, k], wx)
+
+
 def test_singular_pivot_is_reported(dev):
     rng, n, rp, ci = _random_case(3, n=40)
     plan = pv.Plan(n, rp, ci)
