@@ -427,7 +427,15 @@ void compile_fast_programs(pvglu_plan *P) {
       P->k_lu, P->n, 16, [&](int c) { return lu_column_qualifies(P, c); },
       [&](int c, int prev, int *r) { encode_lu_record(P, c, prev, r); },
       P->fast_lu_rec, P->fast_lu_run, P->fast_lu_chain,
-      [&](int c) { return (P->col_flags[c] & 2) ? c : -1; });
+      [&](int c) {
+        // the kernels leave a generic column's last four entries in the
+        // window; such a column has at least four (a column failing the
+        // register path with fewer would need more than two fused forward
+        // terms, i.e. at least three U entries), and the guard keeps the
+        // encoding consistent whatever the pattern
+        return ((P->col_flags[c] & 2) && P->cp[c + 1] - P->cp[c] >= 4) ? c
+                                                                       : -1;
+      });
   // star columns (never on the register path: more than four entries or
   // forward terms) get their 16-int record at their position: q0, q1,
   // diagonal slot, flags, forward row range, update count, then (first,
