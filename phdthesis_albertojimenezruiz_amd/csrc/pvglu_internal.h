@@ -10,6 +10,11 @@
 
 namespace pvglu {
 
+// Star columns (updates only on the diagonal, strided operands) take at most
+// this many updates: the star path's register budget in the narrow-run
+// kernel, and the plan's limit for flagging a column as a star.
+constexpr int kStarMaxUpdates = 32;
+
 // Thread-local error message + status helper.
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();

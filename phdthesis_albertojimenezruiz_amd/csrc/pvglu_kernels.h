@@ -22,6 +22,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "pvglu_internal.h"
 #include "pvglu_launch.h"
 
 #include <cmath>
@@ -533,7 +534,7 @@ __device__ __forceinline__ void tri_lu_segment(
 // column costs one memory round trip instead of one per operand chunk.  Used
 // by the narrow-run kernel, where the column is on a latency-bound chain.
 // Operation order per entry = refactor_column's.
-constexpr int kStarMax = 32;
+constexpr int kStarMax = kStarMaxUpdates;
 
 // The star column's 16-int record (built at plan time, at the column's
 // position in kernel order): q0, q1, diagonal slot, column flags, forward row

@@ -253,9 +253,6 @@ void build_kernel_levels(const LevelSet &ls, const std::vector<int> &comp_of,
 // run[t] = end position of the register run that contains t (0 if t does not
 // qualify); chain flag = 1 when one run covers the whole chain.
 static constexpr int kFastMaxE = 4, kFastMaxOps = 4, kFastMaxLr = 2;
-// star columns: at most this many updates on the diagonal (the kernel's
-// register budget for the star path)
-static constexpr int kStarMaxOps = 32;
 
 // With the fused forward substitution the record also carries the L row of
 // the column (at most kFastMaxLr terms); without it the row is left out.
@@ -721,7 +718,7 @@ extern "C" int pvglu_plan_create(int n, const int *rowptr, const int *colidx,
   }
   // Column flags.  Bit 0: the diagonal updates and the forward-substitution
   // row use the same L operands in the same order (both sums in one pass).
-  // Bit 1: a star column -- only the diagonal has updates (1 to kStarMaxOps
+  // Bit 1: a star column -- only the diagonal has updates (1 to kStarMaxUpdates
   // of them), so every U operand is an entry above the diagonal without
   // updates, i.e. its A value, and the update operands are arithmetic
   // progressions: L slots, U slots and (fused row) forward rows each with a
@@ -740,7 +737,7 @@ extern "C" int pvglu_plan_create(int n, const int *rowptr, const int *colidx,
       P->col_flags[c] |= same ? 1 : 0;
     }
     const int q0 = P->cp[c], q1 = P->cp[c + 1];
-    bool star = o1 - o0 >= 1 && o1 - o0 <= kStarMaxOps &&
+    bool star = o1 - o0 >= 1 && o1 - o0 <= kStarMaxUpdates &&
                 P->op_ptr[q1] - P->op_ptr[q0] == o1 - o0;
     auto progression = [&](const int *v, int len) {
       for (int k = 2; k < len; k++)
