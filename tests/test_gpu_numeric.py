@@ -176,7 +176,8 @@ def test_component_groups_bitexact(dev, variant, groups, width):
 
 # (variant, width, component groups on side streams)
 PLANT_SCHEDS = [(pv.SCHED_AUTO, 0, 1), (pv.SCHED_LEVEL, 0, 1), (pv.SCHED_COMPONENT, 0, 1),
-                (pv.SCHED_HYBRID, 0, 1), (pv.SCHED_LEVEL, 0, 2), (pv.SCHED_HYBRID, 0, 4)]
+                (pv.SCHED_HYBRID, 0, 1), (pv.SCHED_LEVEL, 0, 2), (pv.SCHED_HYBRID, 0, 2),
+                (pv.SCHED_HYBRID, 0, 4)]
 
 
 @pytest.mark.parametrize("cfg", [(4, 10, 20), (40, 24, 28)])
