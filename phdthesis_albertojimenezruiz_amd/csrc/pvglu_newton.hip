@@ -609,6 +609,13 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
                      d->iters, d->lastmax);
   if ((st = build_graph(pv, B)))
     return st;
+  // The host polls the remaining-instance counter after every kPoll graph
+  // replays: a replay after an instance has converged leaves it unchanged
+  // (frozen: no evaluation, stamping or update, and the refactor + solve of
+  // its unchanged system reproduces its solution bit for bit), so the
+  // extra replays of a burst cost time only, and the GPU no longer idles for
+  // a host round trip after each iteration.
+  constexpr int kPoll = 4;
   int iters = 0, remaining = 1;
   auto t0 = std::chrono::steady_clock::now();
   while (remaining > 0) {
@@ -619,8 +626,10 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
       return fail(PVGLU_E_NOCONV, "batched Newton: %d iterations, %d instances left",
                   iters, remaining);
     }
-    HIP_TRY(hipGraphLaunch(d->exec, d->s));
-    iters++;
+    const int burst = std::min(kPoll, max_iters - iters);
+    for (int k = 0; k < burst; ++k)
+      HIP_TRY(hipGraphLaunch(d->exec, d->s));
+    iters += burst;
     HIP_TRY(hipMemcpyAsync(&remaining, d->remaining, sizeof(int),
                            hipMemcpyDeviceToHost, d->s));
     HIP_TRY(hipStreamSynchronize(d->s));
