@@ -241,6 +241,9 @@ def main():
         v, w, g = args.schedule.split(":")
         plan.set_schedule(SCHED_NAMES[v], int(w))
         plan.set_stream_groups(int(g))
+        # the cache policy by its rule (the batched Newton's autotune may have
+        # left its own pick on the plan): streaming for this 1.2 GB batch
+        plan.set_cache_policy(pv.CACHE_AUTO)
     sched = (*plan.get_schedule(), plan.get_stream_groups())
     sched_name = {1: "level", 2: "component", 3: "hybrid"}[sched[0]] + f":{sched[1]}:{sched[2]}"
 
