@@ -39,7 +39,7 @@ constexpr int kStride = kCloudParams + 2; // params, time, clear-sky flag
 __global__ void __launch_bounds__(256)
     k_clouds(int nmod, int B, const double *__restrict__ pos,
              const double *__restrict__ prm, double *__restrict__ g) {
-  PVGLU_ROW_INDEX(B);
+  PVGLU_ROW_INDEX(nmod, B);
   const int m = row_;
   const double *p = prm + (size_t)inst * kStride;
   if (p[kCloudParams + 1] != 0.0) { // uniform 1000 W/m^2

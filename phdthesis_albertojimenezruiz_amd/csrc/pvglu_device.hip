@@ -93,58 +93,93 @@ __global__ void k_fill_int(int *p, int n, int v) {
 }
 
 // ----------------------------------------------------------- 011 update
-// One thread per (element chunk, instance); Σ|Δ| is accumulated with atomics
-// (order-dependent in the last bits; the maximum is exact).  Per element the
-// damping factor
-// halves until |candidate - old| <= 1 V and nothing is NaN
-// (is_value_change_justified, PV/011_calculate_new_iteration.cpp:6-17); the
-// reference's loop halves each unjustified element's factor independently and
-// aborts once any factor is below 1e-13 (:44-74).
+// Per element the damping factor halves until |candidate - old| <= 1 V and
+// nothing is NaN (is_value_change_justified,
+// PV/011_calculate_new_iteration.cpp:6-17); the reference's loop halves each
+// unjustified element's factor independently and aborts once any factor is
+// below 1e-13 (:44-74).
+//
+// Launch shape: a 256-thread block holds TW instance lanes (TW = the power of
+// two >= B, at most 256) by RP = 256/TW element rows and covers
+// RP * kUpdPerThread consecutive elements, row r taking e0 + r, e0 + r + RP,
+// ...; the loads of kUpdBatch elements are issued together.  Per instance the
+// block's partial sums and maxima are combined in LDS in a fixed order and
+// added to the totals with one atomic each (the sum is order-dependent in the
+// last bits across blocks; the maximum is exact).
+constexpr int kUpdPerThread = 16;
+constexpr int kUpdBatch = 4;
+
+__device__ __forceinline__ int upd_tile_log2(int B) {
+  return B >= 256 ? 8 : (B <= 1 ? 0 : 32 - __builtin_clz((unsigned)(B - 1)));
+}
+
 __global__ void __launch_bounds__(256)
-    k_newton_update(int ne, int B, int per_thread, const int *__restrict__ pos,
+    k_newton_update(int ne, int B, const int *__restrict__ pos,
                     const int *__restrict__ neg, const double *__restrict__ x,
                     double *__restrict__ v, double *delta_sum,
                     unsigned long long *delta_max, const int *active,
                     int *status) {
-  // block -> (element chunk, tile of up to 256 instances)
-  const int chunk = (int)blockIdx.x;
-  const int in = (int)(blockIdx.y * blockDim.x + threadIdx.x);
-  if (in >= B)
-    return;
-  if (active && !active[in])
-    return; // converged or stalled instance: frozen
-  const int e0 = chunk * per_thread;
-  const int e1 = min(ne, e0 + per_thread);
+  __shared__ double s_sum[256], s_max[256];
+  const int lt = upd_tile_log2(B), rp = 256 >> lt;
+  const int lane = (int)(threadIdx.x & ((1u << lt) - 1u));
+  const int r = (int)(threadIdx.x >> lt);
+  const int in = (int)(blockIdx.y << lt) + lane;
+  const bool live = in < B && !(active && !active[in]);
+  const int e0 = (int)blockIdx.x * rp * kUpdPerThread;
+  const int e1 = min(ne, e0 + rp * kUpdPerThread);
   double sum = 0.0, mx = 0.0;
-  for (int e = e0; e < e1; ++e) {
-    const int p = pos[e], m_ = neg[e];
-    const double vp = p >= 0 ? x[(size_t)p * B + in] : 0.0;
-    const double vn = m_ >= 0 ? x[(size_t)m_ * B + in] : 0.0;
-    const double nw = vp - vn; // voltage_drop_dc (PV/002_base_electric_system.cpp:21-33)
-    const double old = v[(size_t)e * B + in];
-    double m = 1.0, cand = old;
-    bool ok = false;
-    for (int k = 0; k < 64; ++k) {
-      if (m < 1e-13)
-        break;
-      cand = old * (1.0 - m) + nw * m;
-      if (!isnan(old) && !isnan(cand) && !(fabs(old - cand) > 1.0)) {
-        ok = true;
-        break;
+  if (live) {
+    for (int eb = e0 + r; eb < e1; eb += kUpdBatch * rp) {
+      double nw[kUpdBatch], old[kUpdBatch];
+#pragma unroll
+      for (int j = 0; j < kUpdBatch; ++j) {
+        const int e = min(eb + j * rp, e1 - 1); // in bounds; unused if past e1
+        const int p = pos[e], m_ = neg[e];
+        const double vp = p >= 0 ? x[(size_t)p * B + in] : 0.0;
+        const double vn = m_ >= 0 ? x[(size_t)m_ * B + in] : 0.0;
+        nw[j] = vp - vn; // voltage_drop_dc (PV/002_base_electric_system.cpp:21-33)
+        old[j] = v[(size_t)e * B + in];
       }
-      m = m / 2.0;
+#pragma unroll
+      for (int j = 0; j < kUpdBatch; ++j) {
+        const int e = eb + j * rp;
+        if (e >= e1)
+          break;
+        double m = 1.0, cand = old[j];
+        bool ok = false;
+        for (int k = 0; k < 64; ++k) {
+          if (m < 1e-13)
+            break;
+          cand = old[j] * (1.0 - m) + nw[j] * m;
+          if (!isnan(old[j]) && !isnan(cand) && !(fabs(old[j] - cand) > 1.0)) {
+            ok = true;
+            break;
+          }
+          m = m / 2.0;
+        }
+        if (!ok) {
+          record_status(status, PVGLU_E_NOCONV, e, in);
+          cand = old[j];
+        }
+        const double d = fabs(cand - old[j]);
+        sum += d;
+        mx = fmax(mx, d);
+        v[(size_t)e * B + in] = cand;
+      }
     }
-    if (!ok) {
-      record_status(status, PVGLU_E_NOCONV, e, in);
-      cand = old;
-    }
-    const double d = fabs(cand - old);
-    sum += d;
-    mx = fmax(mx, d);
-    v[(size_t)e * B + in] = cand;
   }
-  atomicAdd(&delta_sum[in], sum);
-  atomicMax(&delta_max[in], (unsigned long long)__double_as_longlong(mx));
+  s_sum[threadIdx.x] = sum;
+  s_max[threadIdx.x] = mx;
+  __syncthreads();
+  if (r == 0 && live) {
+    double ts = 0.0, tm = 0.0;
+    for (int k = 0; k < rp; ++k) {
+      ts += s_sum[(k << lt) + lane];
+      tm = fmax(tm, s_max[(k << lt) + lane]);
+    }
+    atomicAdd(&delta_sum[in], ts);
+    atomicMax(&delta_max[in], (unsigned long long)__double_as_longlong(tm));
+  }
 }
 
 // -------------------------------------------------------------- host side
@@ -751,17 +786,19 @@ extern "C" int pvglu_newton_update(pvglu_plan *P, int ne, int batch,
   if (ne <= 0 || batch <= 0 || !pos || !neg || !x || !v_elem || !delta_sum ||
       !delta_max)
     return fail(PVGLU_E_ARG, "newton_update: bad arguments");
-  const int per = 32;
-  const int chunks = (ne + per - 1) / per;
-  const int bs = batch >= 256 ? 256 : ((batch + 63) / 64) * 64;
+  const int lt = batch >= 256
+                     ? 8
+                     : (batch <= 1 ? 0 : 32 - __builtin_clz((unsigned)(batch - 1)));
+  const int per_block = (256 >> lt) * kUpdPerThread;
+  const int chunks = (ne + per_block - 1) / per_block;
   // zero the per-instance reductions (fill kernel: capture-safe)
   hipLaunchKernelGGL(k_fill_int, dim3((2 * batch + 255) / 256), dim3(256), 0,
                      (hipStream_t)stream, (int *)delta_sum, 2 * batch, 0);
   hipLaunchKernelGGL(k_fill_int, dim3((2 * batch + 255) / 256), dim3(256), 0,
                      (hipStream_t)stream, (int *)delta_max, 2 * batch, 0);
-  hipLaunchKernelGGL(k_newton_update, dim3(chunks, (batch + bs - 1) / bs),
-                     dim3(bs), 0,
-                     (hipStream_t)stream, ne, batch, per, pos, neg, x, v_elem,
+  hipLaunchKernelGGL(k_newton_update,
+                     dim3(chunks, (batch + (1 << lt) - 1) >> lt), dim3(256), 0,
+                     (hipStream_t)stream, ne, batch, pos, neg, x, v_elem,
                      delta_sum, (unsigned long long *)delta_max, active,
                      P->dev->status);
   HIP_TRY(hipGetLastError());

@@ -66,21 +66,31 @@ PVGLU_HD double lambertw_exp(double x) {
 }
 
 #ifdef __HIPCC__
-// Launch shape of the [row][instance] kernels: one block covers up to 256
-// instances of one row (grid.x = rows, grid.y = instance tiles), so a
-// thread's row and instance come from the block indices without a 64-bit
-// division and the row's metadata is uniform in the block.
+// Launch shape of the [row][instance] kernels: a 256-thread block holds TW
+// instance lanes (TW = the power of two >= B, at most 256) by 256/TW rows,
+// so a wavefront's lanes all have work at any batch size (consecutive lanes:
+// consecutive instances, then the next row) and a launch has about
+// rows * B / 256 blocks.  (One block per row leaves most lanes idle when B is
+// small, and hundreds of thousands of tiny blocks make the launch
+// dispatch-bound.)
 struct RowGrid {
   dim3 grid, block;
 };
-inline RowGrid row_grid(long rows, int B) {
-  const int bs = B >= 256 ? 256 : ((B + 63) / 64) * 64;
-  return {dim3((unsigned)rows, (unsigned)((B + bs - 1) / bs)), dim3(bs)};
+PVGLU_HD int row_tile_log2(int B) {
+  return B >= 256 ? 8 : (B <= 1 ? 0 : 32 - __builtin_clz((unsigned)(B - 1)));
 }
-#define PVGLU_ROW_INDEX(B)                                                     \
-  const int row_ = (int)blockIdx.x;                                            \
-  const int inst = (int)(blockIdx.y * blockDim.x + threadIdx.x);               \
-  if (inst >= (B))                                                             \
+inline RowGrid row_grid(long rows, int B) {
+  const int lt = row_tile_log2(B), rp = 256 >> lt;
+  return {dim3((unsigned)((rows + rp - 1) / rp),
+               (unsigned)((B + (1 << lt) - 1) >> lt)),
+          dim3(256)};
+}
+#define PVGLU_ROW_INDEX(ROWS, B)                                               \
+  const int lt_ = pvglu::row_tile_log2(B);                                     \
+  const int row_ = (int)((blockIdx.x << (8 - lt_)) + (threadIdx.x >> lt_));    \
+  const int inst =                                                             \
+      (int)((blockIdx.y << lt_) + (threadIdx.x & ((1u << lt_) - 1u)));         \
+  if (inst >= (B) || row_ >= (ROWS))                                           \
     return;                                                                    \
   const size_t gid = (size_t)row_ * (size_t)(B) + (size_t)inst
 #endif
@@ -89,17 +99,29 @@ struct ModuleEval {
   double I, dIdV, dIdIph;
 };
 
-PVGLU_HD ModuleEval module_current(double V, double Iph, double Rsh) {
+// The per-module constant of ln theta: ln(Rs Rsh I0 / (a (Rs + Rsh))).
+PVGLU_HD double module_log_k(double Rsh) {
+  const double Rs = kModuleRs, I0 = kModuleI0, a = kModuleA;
+  return log(Rs * Rsh * I0 / (a * (Rs + Rsh)));
+}
+
+// module_current with the per-module constant lk = module_log_k(Rsh) given
+// (computed once per module by the batched evaluators; the same operations).
+PVGLU_HD ModuleEval module_current_lk(double V, double Iph, double Rsh,
+                                      double lk) {
   const double Rs = kModuleRs, I0 = kModuleI0, a = kModuleA;
   const double RsRsh = Rs + Rsh;
-  const double x = log(Rs * Rsh * I0 / (a * RsRsh)) +
-                   Rsh * (Rs * (Iph + I0) + V) / (a * RsRsh);
+  const double x = lk + Rsh * (Rs * (Iph + I0) + V) / (a * RsRsh);
   const double W = lambertw_exp(x);
   ModuleEval e;
   e.I = (Rsh * (Iph + I0) - V) / RsRsh - (a / Rs) * W;
   e.dIdV = -1.0 / RsRsh - Rsh / (Rs * RsRsh) * (W / (1.0 + W));
   e.dIdIph = (Rsh / RsRsh) / (1.0 + W);
   return e;
+}
+
+PVGLU_HD ModuleEval module_current(double V, double Iph, double Rsh) {
+  return module_current_lk(V, Iph, Rsh, module_log_k(Rsh));
 }
 
 } // namespace pvglu

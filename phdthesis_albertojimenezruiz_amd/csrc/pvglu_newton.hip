@@ -59,7 +59,7 @@ struct NewtonDev {
   int B = 0;
   // pattern-side (shared by instances)
   double *base_a = nullptr, *base_b = nullptr, *rsh = nullptr,
-         *cab_g = nullptr;
+         *cab_g = nullptr, *lk = nullptr; // lk: module_log_k per module
   int *a_ptr = nullptr, *a_mod = nullptr, *a_sgn = nullptr, *b_ptr = nullptr,
       *b_mod = nullptr, *b_sgn = nullptr, *pos = nullptr, *neg = nullptr,
       *rpos = nullptr, *rneg = nullptr, *cab_ra = nullptr, *cab_rb = nullptr,
@@ -105,24 +105,27 @@ void newton_dev_destroy(NewtonDev *d) {
 // element current f(V) = -I(V) flows + -> -, G = f'(V), I_eq = f(V) - G V.
 __global__ void k_eval(int nm, int B, const double *__restrict__ Girr,
                        const double *__restrict__ rsh,
+                       const double *__restrict__ lk,
                        const double *__restrict__ v, double *__restrict__ geq,
                        double *__restrict__ ieq, const int *active) {
-  PVGLU_ROW_INDEX(B);
+  PVGLU_ROW_INDEX(nm, B);
   if (active && !active[inst])
     return; // frozen instance: its system (and so its solution) stays as is
   const int e = row_;
   const double V = v[gid];
-  const ModuleEval me = module_current(V, kIphPerIrradiance * Girr[gid], rsh[e]);
+  const ModuleEval me =
+      module_current_lk(V, kIphPerIrradiance * Girr[gid], rsh[e], lk[e]);
   const double f = -me.I, g = -me.dIdV;
   geq[gid] = g;
   ieq[gid] = f - g * V;
 }
 
-// Stamping gathers.  A block covers kStampRows consecutive rows (slots of A,
-// or rows of b) for up to 256 instances.  When every row of the block has at
-// most two contributing modules (all rows of a PV plant), the block loads all
-// its module values first and then combines them row by row, so a block is
-// one memory round trip; otherwise the rows are combined one at a time.  The
+// Stamping gathers.  A thread covers kStampRows consecutive rows (slots of A,
+// or rows of b) of one instance; the row groups are laid out as the rows of
+// row_grid.  When every row of the group has at most two contributing modules
+// (all rows of a PV plant), the thread loads all its module values first and
+// then combines them row by row, so a group is one memory round trip;
+// otherwise the rows are combined one at a time.  The
 // operation order per row is fixed (contributions in map order).
 constexpr int kStampRows = 8;
 
@@ -136,8 +139,7 @@ __device__ __forceinline__ void stamp_rows(int rows, int B,
                                            const int *__restrict__ sgn,
                                            const double *__restrict__ val,
                                            double gv, double *__restrict__ out,
-                                           int inst) {
-  const int r0 = (int)blockIdx.x * kStampRows;
+                                           int r0, int inst) {
   int kp[kStampRows + 1];
 #pragma unroll
   for (int r = 0; r <= kStampRows; ++r)
@@ -198,11 +200,13 @@ __global__ void __launch_bounds__(256)
               const int *__restrict__ sgn, const double *__restrict__ geq,
               const double *__restrict__ gvp, double *__restrict__ A,
               const int *active) {
-  const int inst = (int)(blockIdx.y * blockDim.x + threadIdx.x);
-  if (inst >= B || (active && !active[inst]))
+  PVGLU_ROW_INDEX((nnz + kStampRows - 1) / kStampRows, B);
+  (void)gid;
+  if (active && !active[inst])
     return;
   // gvp: homotopy conductance of this instance's stage
-  stamp_rows<false>(nnz, B, base, ptr, mod, sgn, geq, gvp[inst], A, inst);
+  stamp_rows<false>(nnz, B, base, ptr, mod, sgn, geq, gvp[inst], A,
+                    row_ * kStampRows, inst);
 }
 
 // b[r] = linear sources - sum over modules of sign * I_eq
@@ -211,10 +215,12 @@ __global__ void __launch_bounds__(256)
               const int *__restrict__ ptr, const int *__restrict__ mod,
               const int *__restrict__ sgn, const double *__restrict__ ieq,
               double *__restrict__ b, const int *active) {
-  const int inst = (int)(blockIdx.y * blockDim.x + threadIdx.x);
-  if (inst >= B || (active && !active[inst]))
+  PVGLU_ROW_INDEX((n + kStampRows - 1) / kStampRows, B);
+  (void)gid;
+  if (active && !active[inst])
     return;
-  stamp_rows<true>(n, B, base, ptr, mod, sgn, ieq, 0.0, b, inst);
+  stamp_rows<true>(n, B, base, ptr, mod, sgn, ieq, 0.0, b, row_ * kStampRows,
+                   inst);
 }
 
 // dP/dx on the bus unknowns (permuted): P = sum_bus V G_N (V - 600);
@@ -227,7 +233,8 @@ __global__ void k_zero(long count, double *p) {
 
 __global__ void k_power_grad_bus(int nbus, int B, const int *__restrict__ bus_c,
                                  const double *__restrict__ x, double *gP) {
-  PVGLU_ROW_INDEX(B);
+  PVGLU_ROW_INDEX(nbus, B);
+  (void)gid;
   const int k = row_;
   const size_t idx = (size_t)bus_c[k] * B + inst;
   gP[idx] = kNortonG * (2.0 * x[idx] - kNortonV);
@@ -240,7 +247,7 @@ __global__ void k_grad_modules(int nm, int B, const double *__restrict__ Girr,
                                const int *__restrict__ rpos,
                                const int *__restrict__ rneg,
                                const double *__restrict__ mu, double *dG) {
-  PVGLU_ROW_INDEX(B);
+  PVGLU_ROW_INDEX(nm, B);
   const int e = row_;
   const ModuleEval me =
       module_current(v[gid], kIphPerIrradiance * Girr[gid], rsh[e]);
@@ -256,13 +263,22 @@ __global__ void k_grad_cables(int nc, int B, const int *__restrict__ ra,
                               const int *__restrict__ cb,
                               const double *__restrict__ x,
                               const double *__restrict__ mu, double *dg) {
-  PVGLU_ROW_INDEX(B);
+  PVGLU_ROW_INDEX(nc, B);
   const int c = row_;
   const double dv = x[(size_t)ca[c] * B + inst] - x[(size_t)cb[c] * B + inst];
   const double dl = mu[(size_t)ra[c] * B + inst] - mu[(size_t)rb[c] * B + inst];
   dg[gid] = -dl * dv;
 }
 
+
+// module_log_k per module, once per plant (the evaluators then skip one
+// logarithm per module and instance)
+__global__ void k_module_lk(int nm, const double *__restrict__ rsh,
+                            double *__restrict__ lk) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nm)
+    lk[e] = module_log_k(rsh[e]);
+}
 
 __global__ void k_fill_int_n(int *p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -467,9 +483,16 @@ int newton_setup(pvglu_pv *pv) {
       (st = dupload(A, &d->cab_ca, cca)) || (st = dupload(A, &d->cab_cb, ccb)) ||
       (st = dupload(A, &d->bus_c, busc)) ||
       (st = dupload(A, &d->ladder,
-                    std::vector<double>(kLadder, kLadder + 12))))
+                    std::vector<double>(kLadder, kLadder + 12))) ||
+      (st = dupload(A, &d->lk, std::vector<double>(pv->nmod, 0.0))))
     return st;
   HIP_TRY(hipStreamCreate(&d->s));
+  if (pv->nmod > 0) {
+    hipLaunchKernelGGL(k_module_lk, dim3((pv->nmod + 255) / 256), dim3(256), 0,
+                       d->s, pv->nmod, d->rsh, d->lk);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(d->s));
+  }
   return PVGLU_OK;
 }
 
@@ -518,7 +541,7 @@ int launch_stamp(pvglu_pv *pv, int B, const int *active) {
                 ga = row_grid((nnz + kStampRows - 1) / kStampRows, B),
                 gb = row_grid((n + kStampRows - 1) / kStampRows, B);
   hipLaunchKernelGGL(k_eval, ge.grid, ge.block, 0, d->s, nm, B, d->G, d->rsh,
-                     d->v, d->geq, d->ieq, active);
+                     d->lk, d->v, d->geq, d->ieq, active);
   hipLaunchKernelGGL(k_stamp_a, ga.grid, ga.block, 0, d->s, nnz, B, d->base_a,
                      d->a_ptr, d->a_mod, d->a_sgn, d->geq, d->gvs, d->A, active);
   hipLaunchKernelGGL(k_stamp_b, gb.grid, gb.block, 0, d->s, n, B, d->base_b,
@@ -908,8 +931,8 @@ extern "C" int pvglu_pv_stamp(pvglu_pv *pv, int batch, const double *g,
   const RowGrid ge = row_grid(nm, B),
                 ga = row_grid((nnz + kStampRows - 1) / kStampRows, B),
                 gb = row_grid((n + kStampRows - 1) / kStampRows, B);
-  hipLaunchKernelGGL(k_eval, ge.grid, ge.block, 0, s, nm, B, g, d->rsh, v, geq,
-                     ieq, nullptr);
+  hipLaunchKernelGGL(k_eval, ge.grid, ge.block, 0, s, nm, B, g, d->rsh, d->lk,
+                     v, geq, ieq, nullptr);
   hipLaunchKernelGGL(k_stamp_a, ga.grid, ga.block, 0, s, nnz, B, d->base_a,
                      d->a_ptr, d->a_mod, d->a_sgn, geq, d->gvs, a, nullptr);
   hipLaunchKernelGGL(k_stamp_b, gb.grid, gb.block, 0, s, n, B, d->base_b,

@@ -214,13 +214,15 @@ def plant_perm(plant):
     return pv.reorder(plant.n, rp, ci)
 
 
-def test_device_stamping_matches_oracle_assembly(cfg0, dev):
+@pytest.mark.parametrize("B", [3, 32, 300])
+def test_device_stamping_matches_oracle_assembly(cfg0, dev, B):
     """pvglu_pv_stamp against the oracle's create_matrix_and_vector +
     permutation + CSC value fill, at random module voltages (incl. reverse
-    bias), with and without the homotopy conductance."""
+    bias), with and without the homotopy conductance; batches of 3, 32 and
+    300 instances: instance tiles of 4 lanes (3 used), 32 (all used) and 256
+    (plus a partial second tile)."""
     plant = cfg0
     on = _oracle(plant)
-    B = 3
     rng = np.random.default_rng(5)
     G = np.stack([plant.irradiance(s, 0.0) for s in range(B)], axis=1)
     V = rng.uniform(-60.0, 40.0, (plant.nmod, B))
@@ -231,7 +233,7 @@ def test_device_stamping_matches_oracle_assembly(cfg0, dev):
         plant.stamp(B, _t(G, dev), _t(V, dev), A, b, gv)
         torch.cuda.synchronize()
         Ah, bh = A.cpu().numpy(), b.cpu().numpy()
-        for k in range(B):
+        for k in sorted({0, 1, B // 2, B - 1}):
             vals, bp = on.assemble(V[:, k], 9e-3 * G[:, k], gv)
             assert np.allclose(Ah[:, k], vals, rtol=1e-12, atol=1e-12), (gv, k)
             assert np.allclose(bh[:, k], bp, rtol=1e-12, atol=1e-9), (gv, k)

@@ -402,9 +402,12 @@ def test_singular_pivot_is_reported(dev):
     assert plan.status()[0] == 0
 
 
-def test_newton_update_matches_oracle(dev):
+@pytest.mark.parametrize("n,ne,B", [(50, 80, 33), (40, 1000, 1), (40, 700, 32),
+                                    (30, 90, 300)])
+def test_newton_update_matches_oracle(dev, n, ne, B):
+    # instance tiles of 64 (B=33: half-empty), 1, 32 and 256 + a partial one;
+    # element counts that end inside a block
     rng = np.random.default_rng(4)
-    n, ne, B = 50, 80, 33
     pos = rng.integers(-1, n, ne).astype(np.int32)
     neg = rng.integers(-1, n, ne).astype(np.int32)
     X = rng.normal(scale=3.0, size=(n, B))
