@@ -93,93 +93,16 @@ __global__ void k_fill_int(int *p, int n, int v) {
 }
 
 // ----------------------------------------------------------- 011 update
-// Per element the damping factor halves until |candidate - old| <= 1 V and
-// nothing is NaN (is_value_change_justified,
-// PV/011_calculate_new_iteration.cpp:6-17); the reference's loop halves each
-// unjustified element's factor independently and aborts once any factor is
-// below 1e-13 (:44-74).
-//
-// Launch shape: a 256-thread block holds TW instance lanes (TW = the power of
-// two >= B, at most 256) by RP = 256/TW element rows and covers
-// RP * kUpdPerThread consecutive elements, row r taking e0 + r, e0 + r + RP,
-// ...; the loads of kUpdBatch elements are issued together.  Per instance the
-// block's partial sums and maxima are combined in LDS in a fixed order and
-// added to the totals with one atomic each (the sum is order-dependent in the
-// last bits across blocks; the maximum is exact).
-constexpr int kUpdPerThread = 16;
-constexpr int kUpdBatch = 4;
-
-__device__ __forceinline__ int upd_tile_log2(int B) {
-  return B >= 256 ? 8 : (B <= 1 ? 0 : 32 - __builtin_clz((unsigned)(B - 1)));
-}
-
+// update_elements (damped_step per element, upd_tile_log2 launch shape) with
+// nothing after each element.
 __global__ void __launch_bounds__(256)
     k_newton_update(int ne, int B, const int *__restrict__ pos,
                     const int *__restrict__ neg, const double *__restrict__ x,
                     double *__restrict__ v, double *delta_sum,
                     unsigned long long *delta_max, const int *active,
                     int *status) {
-  __shared__ double s_sum[256], s_max[256];
-  const int lt = upd_tile_log2(B), rp = 256 >> lt;
-  const int lane = (int)(threadIdx.x & ((1u << lt) - 1u));
-  const int r = (int)(threadIdx.x >> lt);
-  const int in = (int)(blockIdx.y << lt) + lane;
-  const bool live = in < B && !(active && !active[in]);
-  const int e0 = (int)blockIdx.x * rp * kUpdPerThread;
-  const int e1 = min(ne, e0 + rp * kUpdPerThread);
-  double sum = 0.0, mx = 0.0;
-  if (live) {
-    for (int eb = e0 + r; eb < e1; eb += kUpdBatch * rp) {
-      double nw[kUpdBatch], old[kUpdBatch];
-#pragma unroll
-      for (int j = 0; j < kUpdBatch; ++j) {
-        const int e = min(eb + j * rp, e1 - 1); // in bounds; unused if past e1
-        const int p = pos[e], m_ = neg[e];
-        const double vp = p >= 0 ? x[(size_t)p * B + in] : 0.0;
-        const double vn = m_ >= 0 ? x[(size_t)m_ * B + in] : 0.0;
-        nw[j] = vp - vn; // voltage_drop_dc (PV/002_base_electric_system.cpp:21-33)
-        old[j] = v[(size_t)e * B + in];
-      }
-#pragma unroll
-      for (int j = 0; j < kUpdBatch; ++j) {
-        const int e = eb + j * rp;
-        if (e >= e1)
-          break;
-        double m = 1.0, cand = old[j];
-        bool ok = false;
-        for (int k = 0; k < 64; ++k) {
-          if (m < 1e-13)
-            break;
-          cand = old[j] * (1.0 - m) + nw[j] * m;
-          if (!isnan(old[j]) && !isnan(cand) && !(fabs(old[j] - cand) > 1.0)) {
-            ok = true;
-            break;
-          }
-          m = m / 2.0;
-        }
-        if (!ok) {
-          record_status(status, PVGLU_E_NOCONV, e, in);
-          cand = old[j];
-        }
-        const double d = fabs(cand - old[j]);
-        sum += d;
-        mx = fmax(mx, d);
-        v[(size_t)e * B + in] = cand;
-      }
-    }
-  }
-  s_sum[threadIdx.x] = sum;
-  s_max[threadIdx.x] = mx;
-  __syncthreads();
-  if (r == 0 && live) {
-    double ts = 0.0, tm = 0.0;
-    for (int k = 0; k < rp; ++k) {
-      ts += s_sum[(k << lt) + lane];
-      tm = fmax(tm, s_max[(k << lt) + lane]);
-    }
-    atomicAdd(&delta_sum[in], ts);
-    atomicMax(&delta_max[in], (unsigned long long)__double_as_longlong(tm));
-  }
+  update_elements(ne, B, pos, neg, x, v, delta_sum, delta_max, active, status,
+                  [](int, int, double) {});
 }
 
 // -------------------------------------------------------------- host side
@@ -774,6 +697,10 @@ extern "C" int pvglu_get_schedule(const pvglu_plan *P, int *variant,
   return PVGLU_OK;
 }
 
+int *pvglu::device_status(pvglu_plan *P) {
+  return P->dev ? P->dev->status : nullptr;
+}
+
 extern "C" int pvglu_newton_update(pvglu_plan *P, int ne, int batch,
                                    const int *pos, const int *neg,
                                    const double *x, double *v_elem,
@@ -786,9 +713,7 @@ extern "C" int pvglu_newton_update(pvglu_plan *P, int ne, int batch,
   if (ne <= 0 || batch <= 0 || !pos || !neg || !x || !v_elem || !delta_sum ||
       !delta_max)
     return fail(PVGLU_E_ARG, "newton_update: bad arguments");
-  const int lt = batch >= 256
-                     ? 8
-                     : (batch <= 1 ? 0 : 32 - __builtin_clz((unsigned)(batch - 1)));
+  const int lt = upd_tile_log2(batch);
   const int per_block = (256 >> lt) * kUpdPerThread;
   const int chunks = (ne + per_block - 1) / per_block;
   // zero the per-instance reductions (fill kernel: capture-safe)

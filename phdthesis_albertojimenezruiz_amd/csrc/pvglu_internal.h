@@ -117,4 +117,7 @@ void build_kernel_levels(const LevelSet &ls, const std::vector<int> &comp_of,
 void compile_fast_programs(pvglu_plan *P);
 // Device teardown hook (defined in the HIP unit).
 void device_plan_destroy(pvglu_plan *plan);
+// The plan's device status word [code, column, instance] (null before the
+// plan's first device call).
+int *device_status(pvglu_plan *plan);
 } // namespace pvglu

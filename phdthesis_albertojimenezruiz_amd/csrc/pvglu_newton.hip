@@ -15,6 +15,7 @@
 // The adjoint (pvglu_pv_gradients) reuses the factors for A^T mu = dP/dx.
 #include <hip/hip_runtime.h>
 
+#include "pvglu_launch.h"
 #include "pvglu_module.h"
 #include "pvglu_pv.h"
 
@@ -120,14 +121,53 @@ __global__ void k_eval(int nm, int B, const double *__restrict__ Girr,
   ieq[gid] = f - g * V;
 }
 
-// Stamping gathers.  A thread covers kStampRows consecutive rows (slots of A,
-// or rows of b) of one instance; the row groups are laid out as the rows of
-// row_grid.  When every row of the group has at most two contributing modules
-// (all rows of a PV plant), the thread loads all its module values first and
-// then combines them row by row, so a group is one memory round trip;
-// otherwise the rows are combined one at a time.  The
+// 011 update fused with the companion models of the next iteration: once a
+// module's voltage is updated (update_elements), its conductance and current
+// source for the next stamping are evaluated from the new voltage in
+// registers, with k_eval's operations (so the same values), saving k_eval's
+// read of the voltages and one launch per iteration.  An instance that the
+// control kernel freezes right after this update gets an evaluation that
+// nothing reads (stamping skips frozen instances); the control kernel does
+// not change voltages.
+__global__ void __launch_bounds__(256)
+    k_update_eval(int ne, int B, const int *__restrict__ pos,
+                  const int *__restrict__ neg, const double *__restrict__ x,
+                  double *__restrict__ v, double *dsum,
+                  unsigned long long *dmax, const int *active, int *status,
+                  const double *__restrict__ Girr,
+                  const double *__restrict__ rsh,
+                  const double *__restrict__ lk, double *__restrict__ geq,
+                  double *__restrict__ ieq) {
+  update_elements(ne, B, pos, neg, x, v, dsum, dmax, active, status,
+                  [&](int e, int in, double V) {
+                    const size_t gid = (size_t)e * B + in;
+                    const ModuleEval me = module_current_lk(
+                        V, kIphPerIrradiance * Girr[gid], rsh[e], lk[e]);
+                    const double f = -me.I, g = -me.dIdV;
+                    geq[gid] = g;
+                    ieq[gid] = f - g * V;
+                  });
+}
+
+// Stamping gathers.  A block covers kStampRows consecutive rows (slots of A,
+// or rows of b) for up to 256 instances (stamp_grid), so the rows' metadata
+// is uniform in the block and comes through scalar loads.  (The row_grid
+// shape, several row groups per wavefront, fills every lane at small batches
+// but turns those loads into per-lane vector loads, and measured slower: 129
+// vs 104 us for A at 32 instances of the 400-inverter plant.)  When every row
+// of the block has at most two contributing modules (all rows of a PV plant),
+// the block loads all its module values first and then combines them row by
+// row, so a block is one memory round trip; otherwise the rows are combined
+// one at a time.  The
 // operation order per row is fixed (contributions in map order).
 constexpr int kStampRows = 8;
+
+inline RowGrid stamp_grid(long rows, int B) {
+  const int bs = B >= 256 ? 256 : ((B + 63) / 64) * 64;
+  return {dim3((unsigned)((rows + kStampRows - 1) / kStampRows),
+               (unsigned)((B + bs - 1) / bs)),
+          dim3(bs)};
+}
 
 // out[row] = base[row] + sum_k sign_k * val[mod_k] (+ sign_k * gv per term
 // when gv > 0; SUB: minus instead of plus, without gv).
@@ -200,13 +240,12 @@ __global__ void __launch_bounds__(256)
               const int *__restrict__ sgn, const double *__restrict__ geq,
               const double *__restrict__ gvp, double *__restrict__ A,
               const int *active) {
-  PVGLU_ROW_INDEX((nnz + kStampRows - 1) / kStampRows, B);
-  (void)gid;
-  if (active && !active[inst])
+  const int inst = (int)(blockIdx.y * blockDim.x + threadIdx.x);
+  if (inst >= B || (active && !active[inst]))
     return;
   // gvp: homotopy conductance of this instance's stage
   stamp_rows<false>(nnz, B, base, ptr, mod, sgn, geq, gvp[inst], A,
-                    row_ * kStampRows, inst);
+                    (int)blockIdx.x * kStampRows, inst);
 }
 
 // b[r] = linear sources - sum over modules of sign * I_eq
@@ -215,12 +254,11 @@ __global__ void __launch_bounds__(256)
               const int *__restrict__ ptr, const int *__restrict__ mod,
               const int *__restrict__ sgn, const double *__restrict__ ieq,
               double *__restrict__ b, const int *active) {
-  PVGLU_ROW_INDEX((n + kStampRows - 1) / kStampRows, B);
-  (void)gid;
-  if (active && !active[inst])
+  const int inst = (int)(blockIdx.y * blockDim.x + threadIdx.x);
+  if (inst >= B || (active && !active[inst]))
     return;
-  stamp_rows<true>(n, B, base, ptr, mod, sgn, ieq, 0.0, b, row_ * kStampRows,
-                   inst);
+  stamp_rows<true>(n, B, base, ptr, mod, sgn, ieq, 0.0, b,
+                   (int)blockIdx.x * kStampRows, inst);
 }
 
 // dP/dx on the bus unknowns (permuted): P = sum_bus V G_N (V - 600);
@@ -534,14 +572,21 @@ int newton_alloc_instances(pvglu_pv *pv, int B) {
 inline dim3 grid_for(long count) { return dim3((unsigned)((count + 255) / 256)); }
 
 // eval + stamps on stream s
-int launch_stamp(pvglu_pv *pv, int B, const int *active) {
+// Companion models of the instances in `active` (all when null).
+void launch_eval(pvglu_pv *pv, int B, const int *active) {
   NewtonDev *d = pv->nd;
-  const int nm = pv->nmod, n = pv->n, nnz = pv->plan->nnz;
-  const RowGrid ge = row_grid(nm, B),
-                ga = row_grid((nnz + kStampRows - 1) / kStampRows, B),
-                gb = row_grid((n + kStampRows - 1) / kStampRows, B);
-  hipLaunchKernelGGL(k_eval, ge.grid, ge.block, 0, d->s, nm, B, d->G, d->rsh,
-                     d->lk, d->v, d->geq, d->ieq, active);
+  const RowGrid ge = row_grid(pv->nmod, B);
+  hipLaunchKernelGGL(k_eval, ge.grid, ge.block, 0, d->s, pv->nmod, B, d->G,
+                     d->rsh, d->lk, d->v, d->geq, d->ieq, active);
+}
+
+// Stamping of A and b from the companion models; eval: evaluate them first.
+int launch_stamp(pvglu_pv *pv, int B, const int *active, bool eval) {
+  NewtonDev *d = pv->nd;
+  const int n = pv->n, nnz = pv->plan->nnz;
+  const RowGrid ga = stamp_grid(nnz, B), gb = stamp_grid(n, B);
+  if (eval)
+    launch_eval(pv, B, active);
   hipLaunchKernelGGL(k_stamp_a, ga.grid, ga.block, 0, d->s, nnz, B, d->base_a,
                      d->a_ptr, d->a_mod, d->a_sgn, d->geq, d->gvs, d->A, active);
   hipLaunchKernelGGL(k_stamp_b, gb.grid, gb.block, 0, d->s, n, B, d->base_b,
@@ -557,14 +602,28 @@ constexpr int kStallMax = 20;
 int launch_iteration(pvglu_pv *pv, int B) {
   NewtonDev *d = pv->nd;
   hipLaunchKernelGGL(k_fill_int_n, dim3(1), dim3(64), 0, d->s, d->remaining, 1);
-  int st = launch_stamp(pv, B, d->active);
+  // the companion models of this iteration come from the previous
+  // iteration's fused update (or the evaluation before the first replay)
+  int st = launch_stamp(pv, B, d->active, false);
   if (st)
     return st;
   if ((st = pvglu_refactor_solve(pv->plan, d->A, d->LU, d->b, d->x, B, d->s)))
     return st;
-  if ((st = pvglu_newton_update(pv->plan, pv->nmod, B, d->pos, d->neg, d->x,
-                                d->v, d->dsum, d->dmax, d->active, d->s)))
-    return st;
+  int *status = device_status(pv->plan);
+  if (!status)
+    return fail(PVGLU_E_STATE, "batched Newton: plan has no device state");
+  const int nm = pv->nmod, lt = upd_tile_log2(B);
+  const int per_block = (256 >> lt) * kUpdPerThread;
+  hipLaunchKernelGGL(k_fill_double, grid_for(B), dim3(256), 0, d->s, d->dsum,
+                     B, 0.0);
+  hipLaunchKernelGGL(k_fill_double, grid_for(B), dim3(256), 0, d->s, d->dmax,
+                     B, 0.0);
+  hipLaunchKernelGGL(k_update_eval,
+                     dim3((nm + per_block - 1) / per_block,
+                          (B + (1 << lt) - 1) >> lt),
+                     dim3(256), 0, d->s, nm, B, d->pos, d->neg, d->x, d->v,
+                     d->dsum, (unsigned long long *)d->dmax, d->active, status,
+                     d->G, d->rsh, d->lk, d->geq, d->ieq);
   const double *ladder = d->ladder + (12 - d->nstages);
   hipLaunchKernelGGL(k_newton_control, grid_for(B), dim3(256), 0, d->s, B,
                      d->nstages, ladder, 1e-6 * pv->n, d->tol, kNoise, kStallMax,
@@ -630,6 +689,8 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
   hipLaunchKernelGGL(k_newton_init, grid_for(B), dim3(256), 0, d->s, B,
                      kLadder[12 - nst], d->stage, d->gvs, d->active, d->stall,
                      d->iters, d->lastmax);
+  launch_eval(pv, B, d->active); // the first replay's companion models
+  HIP_TRY(hipGetLastError());
   if ((st = build_graph(pv, B)))
     return st;
   // The host polls the remaining-instance counter after every kPoll graph
@@ -734,7 +795,7 @@ extern "C" int pvglu_pv_gradients(pvglu_pv *pv, int batch, const double *g,
   const int B = batch, n = pv->n, nm = pv->nmod, nc = pv->ncab;
   // Jacobian at the converged element voltages: stamp + refactor (the
   // companion matrix at V_e = V_branch(x) is dF/dx), then the last solve's x
-  if ((st = launch_stamp(pv, B, nullptr)))
+  if ((st = launch_stamp(pv, B, nullptr, true)))
     return st;
   if ((st = pvglu_refactor_solve(pv->plan, d->A, d->LU, d->b, d->x, B, d->s)))
     return st;
@@ -929,8 +990,7 @@ extern "C" int pvglu_pv_stamp(pvglu_pv *pv, int batch, const double *g,
   hipLaunchKernelGGL(k_fill_double, grid_for(B), dim3(256), 0, s, d->gvs, B,
                      g_virtual);
   const RowGrid ge = row_grid(nm, B),
-                ga = row_grid((nnz + kStampRows - 1) / kStampRows, B),
-                gb = row_grid((n + kStampRows - 1) / kStampRows, B);
+                ga = stamp_grid(nnz, B), gb = stamp_grid(n, B);
   hipLaunchKernelGGL(k_eval, ge.grid, ge.block, 0, s, nm, B, g, d->rsh, d->lk,
                      v, geq, ieq, nullptr);
   hipLaunchKernelGGL(k_stamp_a, ga.grid, ga.block, 0, s, nnz, B, d->base_a,
