@@ -137,7 +137,10 @@ __global__ void __launch_bounds__(256)
                   const double *__restrict__ Girr,
                   const double *__restrict__ rsh,
                   const double *__restrict__ lk, double *__restrict__ geq,
-                  double *__restrict__ ieq) {
+                  double *__restrict__ ieq, int *remaining) {
+  // the control kernel after this one counts the unfinished instances
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
+    *remaining = 0;
   update_elements(ne, B, pos, neg, x, v, dsum, dmax, active, status,
                   [&](int e, int in, double V) {
                     const size_t gid = (size_t)e * B + in;
@@ -318,12 +321,6 @@ __global__ void k_module_lk(int nm, const double *__restrict__ rsh,
     lk[e] = module_log_k(rsh[e]);
 }
 
-__global__ void k_fill_int_n(int *p, int n) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n)
-    p[i] = 0;
-}
-
 __global__ void k_fill_double(double *p, int n, double v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n)
@@ -333,10 +330,12 @@ __global__ void k_fill_double(double *p, int n, double v) {
 // Per-instance Newton state at the start of a solve.
 __global__ void k_newton_init(int B, double gv0, int *stage, double *gvs,
                               int *active, int *stall, int *iters,
-                              double *lastmax) {
+                              double *lastmax, double *dsum, double *dmax) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= B)
     return;
+  dsum[k] = 0.0; // the update accumulates into these; control clears them
+  dmax[k] = 0.0;
   stage[k] = 0;
   gvs[k] = gv0;
   active[k] = 1;
@@ -354,8 +353,8 @@ __global__ void k_newton_init(int B, double gv0, int *stage, double *gvs,
 __global__ void k_newton_control(int B, int nstages,
                                  const double *__restrict__ ladder,
                                  double stage_crit, double tol, double noise,
-                                 int stall_max, const double *__restrict__ dsum,
-                                 const double *__restrict__ dmax, int *stage,
+                                 int stall_max, double *dsum, double *dmax,
+                                 int *stage,
                                  double *gvs, int *active, int *stall,
                                  int *iters, double *lastmax, int *remaining) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -364,9 +363,12 @@ __global__ void k_newton_control(int B, int nstages,
   iters[k]++;
   const int st = stage[k];
   const double m = dmax[k];
+  const double sum = dsum[k];
+  dsum[k] = 0.0; // cleared for the next iteration's update
+  dmax[k] = 0.0;
   lastmax[k] = m;
   if (st < nstages - 1) {
-    if (dsum[k] < stage_crit) {
+    if (sum < stage_crit) {
       stage[k] = st + 1;
       gvs[k] = ladder[st + 1];
     }
@@ -601,7 +603,6 @@ constexpr int kStallMax = 20;
 
 int launch_iteration(pvglu_pv *pv, int B) {
   NewtonDev *d = pv->nd;
-  hipLaunchKernelGGL(k_fill_int_n, dim3(1), dim3(64), 0, d->s, d->remaining, 1);
   // the companion models of this iteration come from the previous
   // iteration's fused update (or the evaluation before the first replay)
   int st = launch_stamp(pv, B, d->active, false);
@@ -614,16 +615,13 @@ int launch_iteration(pvglu_pv *pv, int B) {
     return fail(PVGLU_E_STATE, "batched Newton: plan has no device state");
   const int nm = pv->nmod, lt = upd_tile_log2(B);
   const int per_block = (256 >> lt) * kUpdPerThread;
-  hipLaunchKernelGGL(k_fill_double, grid_for(B), dim3(256), 0, d->s, d->dsum,
-                     B, 0.0);
-  hipLaunchKernelGGL(k_fill_double, grid_for(B), dim3(256), 0, d->s, d->dmax,
-                     B, 0.0);
+  // dsum / dmax were cleared by k_newton_init or the previous control kernel
   hipLaunchKernelGGL(k_update_eval,
                      dim3((nm + per_block - 1) / per_block,
                           (B + (1 << lt) - 1) >> lt),
                      dim3(256), 0, d->s, nm, B, d->pos, d->neg, d->x, d->v,
                      d->dsum, (unsigned long long *)d->dmax, d->active, status,
-                     d->G, d->rsh, d->lk, d->geq, d->ieq);
+                     d->G, d->rsh, d->lk, d->geq, d->ieq, d->remaining);
   const double *ladder = d->ladder + (12 - d->nstages);
   hipLaunchKernelGGL(k_newton_control, grid_for(B), dim3(256), 0, d->s, B,
                      d->nstages, ladder, 1e-6 * pv->n, d->tol, kNoise, kStallMax,
@@ -688,7 +686,7 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
   d->tol = tol;
   hipLaunchKernelGGL(k_newton_init, grid_for(B), dim3(256), 0, d->s, B,
                      kLadder[12 - nst], d->stage, d->gvs, d->active, d->stall,
-                     d->iters, d->lastmax);
+                     d->iters, d->lastmax, d->dsum, d->dmax);
   launch_eval(pv, B, d->active); // the first replay's companion models
   HIP_TRY(hipGetLastError());
   if ((st = build_graph(pv, B)))
