@@ -9,7 +9,7 @@
 // w = W(e^x) solves w + ln w = x (see lambertw_exp).
 // Derivatives: dI/dV = -1/(Rs+Rsh) - Rsh/(Rs (Rs+Rsh)) * W/(1+W),
 //              dI/dIph = Rsh/(Rs+Rsh) / (1+W).
-// The config modules: Iph = 9 G / 1000 A, I0 = 1e-10 A,
+// The plant modules of the workloads: Iph = 9 G / 1000 A, I0 = 1e-10 A,
 // a = n Vt = 1.5 V, Rs = 0.3 Ohm, Rsh log-uniform 200..600 Ohm.
 #pragma once
 

@@ -1,6 +1,6 @@
 """GPU Newton paths of the config plants against the CPU oracle Newton.
 
-Bar (BASELINE north star): node voltages within 1e-10 V of the FP64 CPU
+Bar: node voltages within 1e-10 V of the FP64 CPU
 oracle's converged solution (oracle.PVNewton: the reference's ladder loop
 with GLU_CSC_CPU_sequential, the CSC substitution and the 011 update, with
 modules solved from the implicit diode equation rather than Lambert W).
