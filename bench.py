@@ -55,27 +55,36 @@ def parse():
     ap.add_argument("--schedule", default="hybrid:64:1",
                     help="variant:width:groups of the batched launch (level, component, "
                          "hybrid), or 'auto' for the autotuner's choice")
-    ap.add_argument("--cpu-threads", type=int, default=16,
-                    help="threads of the all-cores CPU baseline (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the all-cores CPU baseline (0: nproc)")
     return ap.parse_args()
 
 
 SCHED_NAMES = {"level": 1, "component": 2, "hybrid": 3}
 
 
-def _cpu_kind():
+# CPU builds timed as the baseline: the reference's host flags
+# (PV/Makefile:45, -O3 -march=native -funroll-loops --fast-math; built with
+# -march=x86-64-v3 so the library runs on the box's host) and the same
+# without --fast-math.
+CPU_BUILDS = {"fast_math": True, "no_fast_math": "o3"}
+
+
+def _cpu_kind(build=True):
     from oracle import oracle as O
-    return "reference" if O.ref(fast=True) is not None else "port"
+    return "reference" if O.ref(fast=build) is not None else "port"
 
 
-def _cpu_one(s, lu, x, kind):
+def _cpu_one(s, lu, x, kind, build):
     from oracle import oracle as O
     ref = kind == "reference"
-    O.glu_csc_sequential(s, lu, use_ref=ref, fast=True)
-    O.substitute_lu_csc(s, lu, x, use_ref=ref, fast=True)
+    # the port (oracle restatement) has one optimized build only
+    fast = build if ref else True
+    O.glu_csc_sequential(s, lu, use_ref=ref, fast=fast)
+    O.substitute_lu_csc(s, lu, x, use_ref=ref, fast=fast)
 
 
-def _cpu_run(s, AT, bT, ks, kind):
+def _cpu_run(s, AT, bT, ks, kind, build):
     """refactor+solve of instances ks on one thread; each instance's values are
     copied into a scratch vector first (the GLU works in place)."""
     lu = np.empty(AT.shape[1])
@@ -83,35 +92,61 @@ def _cpu_run(s, AT, bT, ks, kind):
     for k in ks:
         np.copyto(lu, AT[k])
         np.copyto(x, bT[k])
-        _cpu_one(s, lu, x, kind)
+        _cpu_one(s, lu, x, kind, build)
 
 
-def cpu_baseline(plan_arrays, A_host, b_host, threads=1, budget_s=12.0):
+def host_cpu():
+    """nproc (the processing units this process may use; GNU nproc honours
+    the affinity mask and OMP_NUM_THREADS), nproc --all and lscpu's model."""
+    def run(cmd):
+        try:
+            return subprocess.run(cmd, capture_output=True, text=True).stdout
+        except OSError:
+            return ""
+    try:
+        nproc = int(run(["nproc"]).strip())
+    except ValueError:
+        nproc = len(os.sched_getaffinity(0))
+    try:
+        nall = int(run(["nproc", "--all"]).strip())
+    except ValueError:
+        nall = os.cpu_count() or nproc
+    model = [l.split(":", 1)[1].strip() for l in run(["lscpu"]).splitlines()
+             if l.startswith("Model name")]
+    return {"nproc": nproc, "nproc_all": nall, "model": model[0] if model else "unknown"}
+
+
+def cpu_baseline(plan_arrays, A_host, b_host, threads=1, budget_s=12.0, build=True):
     """Reference CPU GLU (GLU_CSC_CPU_sequential + Substitute_LU_Vector_b_CSC_CPU
-    compiled from the reference's own sources with its host flags) on a
-    bounded sample of the same instances, on `threads` threads (the foreign
-    calls release the interpreter lock)."""
+    compiled from the reference's own sources; build: True = its host flags
+    with --fast-math, "o3" = without) on a bounded sample of the same
+    instances, on `threads` threads (the foreign calls release the
+    interpreter lock)."""
     from concurrent.futures import ThreadPoolExecutor
     from oracle import oracle as O
-    kind = _cpu_kind()
+    kind = _cpu_kind(build)
     s = O.CsrCsc(*plan_arrays)
     B = A_host.shape[1]
     AT = np.ascontiguousarray(A_host.T)
     bT = np.ascontiguousarray(b_host.T)
-    _cpu_run(s, AT, bT, [k % B for k in range(5)], kind)   # warm-up
+    _cpu_run(s, AT, bT, [k % B for k in range(5)], kind, build)   # warm-up
     t0 = time.perf_counter()
-    _cpu_run(s, AT, bT, [k % B for k in range(20)], kind)
+    _cpu_run(s, AT, bT, [k % B for k in range(20)], kind, build)
     one = max((time.perf_counter() - t0) / 20, 1e-6)
     count = int(min(max(threads, budget_s * threads / one), 400 * B))
     ks = [k % B for k in range(count)]
     t0 = time.perf_counter()
     if threads == 1:
-        _cpu_run(s, AT, bT, ks, kind)
+        _cpu_run(s, AT, bT, ks, kind, build)
     else:
         with ThreadPoolExecutor(threads) as ex:
-            list(ex.map(lambda j: _cpu_run(s, AT, bT, ks[j::threads], kind), range(threads)))
+            list(ex.map(lambda j: _cpu_run(s, AT, bT, ks[j::threads], kind, build),
+                        range(threads)))
     dt = time.perf_counter() - t0
+    flags = ("-O3 -march=x86-64-v3 -funroll-loops" +
+             (" --fast-math" if build is True else ""))
     return {"value": count / dt, "unit": "refactor+solve/s", "cores": threads, "kind": kind,
+            "build": flags if kind == "reference" else "oracle port, -O3 -march=x86-64-v3",
             "sample": f"{count} instances of the batch (refactor+solve each, values copied "
                       f"in per instance), {dt:.1f} s, {threads} thread"
                       + ("s" if threads > 1 else "")}
@@ -312,6 +347,10 @@ def main():
             t = json.load(open(tf))
             if t.get("batch") == B and t.get("n") == n and t.get("schedule") == sched_name:
                 roofline["traffic"] = t.get("bytes_per_launch")
+                roofline["traffic_source"] = (
+                    "calibrated constant from profiles/traffic.json: two rocprofv3 --pmc "
+                    "passes (FETCH_SIZE, WRITE_SIZE) of this workload and schedule, scaled "
+                    "by the counters of a 1 GiB device copy; not measured in this run")
         except (OSError, ValueError):
             pass
     print(f"[bench] batched: {ms_per_step:.4f} ms/step ({avg_launch_ms:.4f} ms/launch), "
@@ -379,24 +418,27 @@ def main():
               f"{st[3] / st[0]:.2f} ms/iteration", flush=True)
 
     if not args.no_cpu:
-        cb = cpu_baseline(plan_arrays, A_host, b_host)
-        try:
-            model = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
-            model = [l.split(":", 1)[1].strip() for l in model.splitlines()
-                     if l.startswith("Model name")][0]
-        except (OSError, IndexError):
-            model = "unknown"
-        cb["cpu"] = model
-        out["cpu_baseline"] = cb
+        host = host_cpu()
+        threads = args.cpu_threads if args.cpu_threads > 0 else host["nproc"]
+        legs = {}
+        for name, build in CPU_BUILDS.items():
+            cb = cpu_baseline(plan_arrays, A_host, b_host, build=build, budget_s=10.0)
+            ct = cpu_baseline(plan_arrays, A_host, b_host, threads=threads, budget_s=6.0,
+                              build=build)
+            legs[name] = (cb, ct)
+            print(f"[bench] cpu baseline {name} ({cb['kind']}): {cb['value']:.4g}/s on 1 "
+                  f"thread, {ct['value']:.4g}/s on {threads} threads ({host['model']}, "
+                  f"nproc {host['nproc']}, nproc --all {host['nproc_all']})", flush=True)
+        cb, ct = legs["fast_math"]
+        out["cpu_baseline"] = dict(cb, cpu=host["model"], nproc=host["nproc"],
+                                   nproc_all=host["nproc_all"])
         out["batched_over_cpu"] = value / cb["value"]
-        print(f"[bench] cpu baseline ({cb['kind']}): {cb['value']:.4g}/s on {model}", flush=True)
-        if args.cpu_threads > 0:
-            ct = cpu_baseline(plan_arrays, A_host, b_host, threads=args.cpu_threads,
-                              budget_s=8.0)
-            ct["cpu"] = model
-            out["cpu_baseline_all_cores"] = ct
-            out["batched_over_cpu_all_cores"] = value / ct["value"]
-            print(f"[bench] cpu baseline, {ct['cores']} threads: {ct['value']:.4g}/s", flush=True)
+        out["cpu_baseline_all_cores"] = dict(ct, cpu=host["model"])
+        out["batched_over_cpu_all_cores"] = value / ct["value"]
+        nb, nt = legs["no_fast_math"]
+        out["cpu_baseline_no_fast_math"] = dict(nb, cpu=host["model"])
+        out["cpu_baseline_no_fast_math_all_cores"] = dict(nt, cpu=host["model"])
+        out["batched_over_cpu_no_fast_math"] = value / nb["value"]
 
     if not args.no_configs:
         out["configs"] = config_legs(torch, pv, dev)

@@ -63,8 +63,8 @@ int pvglu_obtain_fillin(int n, const int *rowptr, const int *colidx,
 
 /* Device prefix sum (inclusive, in place) over n int32 values resident on
  * the device.  Replaces call_cumsum_CUDA (PV/005b_cumsum_CUDA.cu:22-31).
- * Stream-ordered; needs no scratch beyond the array itself for n <= 2^20,
- * larger arrays use a library-held staging buffer. */
+ * Stream-ordered; one workgroup walks the array in 4096-value tiles with a
+ * running carry, so it needs no scratch beyond the array itself. */
 int pvglu_inclusive_scan_device(int *d_values, int n, void *stream);
 
 /* Symbolic plan over a filled pattern: fill_fillins (PV/005c_Rellenar_Fillins.cpp:4-14),
@@ -265,7 +265,10 @@ int pvglu_multi_solve_transpose(pvglu_multi *m, const double *const *lu,
  * pos/neg: permuted row of each element's terminals (-1 = ground);
  * v_elem: element voltages [elem][instance], updated in place.
  * delta_sum / delta_max: per-instance sum and max of |delta| (device,
- * batch entries each).  active (device, batch ints, may be NULL): instances
+ * batch entries each).  The sum is accumulated in 2^-44 V fixed point (each
+ * |delta| <= 1 V rounded down to a multiple of 2^-44 V, integer additions),
+ * so it is the same whatever the block order or batch size; the reference
+ * sums in long double (PV/011_calculate_new_iteration.cpp:39,76-85).  active (device, batch ints, may be NULL): instances
  * with 0 are left untouched (converged).  Damping below 1e-13 records
  * PVGLU_E_NOCONV in the status word of `plan`. */
 int pvglu_newton_update(pvglu_plan *plan, int ne, int batch, const int *pos,
@@ -340,13 +343,36 @@ int pvglu_amd_order(int n, const int *rowptr, const int *colidx, int *P);
 int pvglu_reorder(int n, const int *rowptr, const int *colidx, int *P,
                   int *Prow);
 
+/* Newton step form of the PV drivers below (drop-in, batched, gradients).
+ *   PVGLU_NEWTON_DIRECT      the reference's step: the companion system
+ *                            A(v) x = b(v) solved for the node voltages
+ *                            (PV/000_PVPP_simulator.cpp:185-230 with
+ *                            PV/002b_sistema_electrico_base_matriz.cpp:141-203).
+ *   PVGLU_NEWTON_CORRECTION  (default) the same Newton step written as a
+ *                            correction: A(v) d = b(v) - A(v) x_prev,
+ *                            x = x_prev + d, the right-hand side evaluated
+ *                            branch by branch (KCL residual of the linearized
+ *                            circuit at x_prev; x_prev = 0 at the start).
+ * Both use the same matrices, refactor and solve; in exact arithmetic their
+ * iterates are equal.  In FP64 the direct form's iterates carry the solve's
+ * rounding of the ~600 V node voltages (last-stage noise floor up to ~1e-9 V
+ * behind weak, reverse-biased modules), the correction form's only the
+ * rounding of the branch currents (~1e-13..1e-12 V).  The stall rule of
+ * pvglu_pv_newton_dropin / _batched uses noise level 1e-8 V (direct) or
+ * 1e-11 V (correction).  No counterpart in the reference (its only form is
+ * the direct one). */
+enum { PVGLU_NEWTON_CORRECTION = 0, PVGLU_NEWTON_DIRECT = 1 };
+int pvglu_pv_set_newton_form(pvglu_pv *pv, int form);
+int pvglu_pv_get_newton_form(const pvglu_pv *pv);
+
 /* Host drop-in Newton solve of one plant, with the reference's call sequence
  * (PV/000_PVPP_simulator.cpp:144-235): symbolic setup once, then per
  * iteration host assembly, permutation, CSC value fill, GPU refactor, GPU
  * solve, un-permutation and the 011 update.  g: irradiance per module;
  * v_module: initial/final module voltages (in/out); x_out: node voltages (n).
  * tol: stop when max |delta| <= tol on the last homotopy stage (or after
- * 20 iterations with max |delta| <= 1e-8 V: the FP64 noise floor);
+ * 20 iterations with max |delta| at or below the form's noise level: the
+ * FP64 noise floor);
  * ladder: 1 = the reference's conductance ladder, 0 = last stage only.
  * fixed_iters > 0 runs exactly that many iterations of the last stage.
  * stats[0..3]: iterations, host assembly ms, device ms, total ms. */
@@ -367,7 +393,8 @@ int pvglu_pv_newton_batched(pvglu_pv *pv, int batch, const double *g,
 
 /* Per-instance iteration counts and final max |delta| (V) of the last
  * batched solve.  An instance whose final max |delta| exceeds tol stopped at
- * the FP64 noise floor (20 iterations with max |delta| <= 1e-8 V). */
+ * the FP64 noise floor (20 iterations with max |delta| at or below the
+ * form's noise level). */
 int pvglu_pv_newton_info(const pvglu_pv *pv, int batch, int *iters,
                          double *last_max_delta);
 

@@ -7,6 +7,8 @@
 #   libpvref_fast.so  the reference's own host flags (PV/Makefile:45:
 #                     -O3 -march=native -funroll-loops --fast-math), with
 #                     -march=x86-64-v3 so the library runs on any AVX2 host
+#   libpvref_o3.so    the same without --fast-math (-O3 -march=x86-64-v3
+#                     -funroll-loops): the second CPU baseline
 #
 # Units built: PV/AmdLib/*.c, PV/006_get_dependencies_GLU.cpp,
 # PV/007_convert_dependencies_in_levels.cpp,
@@ -54,4 +56,5 @@ build_variant() {
 
 build_variant "" -O3 -ffp-contract=off
 build_variant "_fast" -O3 -march=x86-64-v3 -funroll-loops --fast-math
-echo "build_ref: $out/libpvref.so $out/libpvref_fast.so"
+build_variant "_o3" -O3 -march=x86-64-v3 -funroll-loops
+echo "build_ref: $out/libpvref.so $out/libpvref_fast.so $out/libpvref_o3.so"

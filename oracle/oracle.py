@@ -50,9 +50,11 @@ def _load(path):
     return L
 
 
-def ref(fast: bool = False):
-    """The reference's own CPU units (None when oracle/_ref was not built)."""
-    name = "libpvref_fast.so" if fast else "libpvref.so"
+def ref(fast=False):
+    """The reference's own CPU units (None when oracle/_ref was not built).
+    fast: False = the parity build (-O3 -ffp-contract=off), True = the
+    reference's host flags with --fast-math, "o3" = the same without it."""
+    name = {False: "libpvref.so", True: "libpvref_fast.so", "o3": "libpvref_o3.so"}[fast]
     if name not in _ref:
         path = os.path.join(HERE, "_ref", name)
         _ref[name] = C.CDLL(path) if os.path.exists(path) else None
@@ -275,14 +277,33 @@ class PVNewton:
     create_matrix_and_vector (PV/002b_sistema_electrico_base_matriz.cpp:141-203),
     symmetric + row permutation, GLU_CSC_CPU_sequential, the CSC substitution,
     un-permutation and the 011 damped update.  Modules use the implicit
-    single-diode solve (orc_module_current), not the library's Lambert W."""
+    single-diode solve (orc_module_current), not the library's Lambert W.
+
+    form="direct" is the reference's iteration: the companion system
+    A(v) x = b(v) is solved for the node voltages themselves.
+    form="correction" takes the same Newton step written as a correction,
+    A(v) d = b(v) - A(v) x_prev, x = x_prev + d, with the right-hand side
+    evaluated branch by branch (the KCL residual of the linearized circuit at
+    x_prev: -sum of each node's branch currents g (x_a - x_b), G_N (x - 600),
+    f(v) + g (V(x_prev) - v) [+ g_virtual V(x_prev)]).  In exact arithmetic
+    both give the same iterates (the first from x_prev = 0 is the direct
+    solve); in FP64 the correction form's rounding scales with |d| instead of
+    |x|, which lowers the last-stage noise floor of shaded strings from
+    ~1e-10 V to ~1e-13 V."""
 
     RS, I0, A = 0.3, 1e-10, 1.5
     GN, VN = 1e4, 600.0
 
-    def __init__(self, n, pos, neg, rsh, cab_a, cab_b, cab_g, buses, P, Prow):
+    def __init__(self, n, pos, neg, rsh, cab_a, cab_b, cab_g, buses, P, Prow, form="direct"):
+        if form not in ("direct", "correction"):
+            raise ValueError(form)
+        self.form = form
         self.n = n
-        self.pos, self.neg, self.rsh = pos, neg, rsh
+        self.pos, self.neg = np.asarray(pos, _I), np.asarray(neg, _I)
+        self.rsh = np.ascontiguousarray(rsh, np.float64)
+        self.cab_a, self.cab_b = np.asarray(cab_a, _I), np.asarray(cab_b, _I)
+        self.cab_g = np.asarray(cab_g, np.float64)
+        self.buses = np.asarray(buses, _I)
         self.P = np.asarray(P, _I)
         Pinv = invert_permutation(self.P)
         self.colmap = Pinv
@@ -341,65 +362,114 @@ class PVNewton:
         self.st_elem = np.array(el)
 
     def module(self, V, iph):
-        out = np.zeros((3, len(V)))
-        for e in range(len(V)):
-            out[:, e] = module_current(V[e], iph[e], self.I0, self.A, self.RS, self.rsh[e])
+        """Rows I, dI/dV, dI/dIph of every module at voltages V."""
+        V = np.ascontiguousarray(V, np.float64)
+        iph = np.ascontiguousarray(iph, np.float64)
+        m = len(V)
+        out = np.zeros((3, m))
+        lib().orc_module_current_many(m, _dp(V), _dp(iph), C.c_double(self.I0),
+                                      C.c_double(self.A), C.c_double(self.RS), _dp(self.rsh),
+                                      _dp(out[0]), _dp(out[1]), _dp(out[2]))
         return out
+
+    def element_voltages(self, x):
+        """V(pos) - V(neg) per module (voltage_drop_dc, PV/002_base_electric_system.cpp:21-33)."""
+        return (np.where(self.pos >= 0, x[np.maximum(self.pos, 0)], 0.0) -
+                np.where(self.neg >= 0, x[np.maximum(self.neg, 0)], 0.0))
+
+    def _values(self, g, gv):
+        vals = self.abase.copy()
+        np.add.at(vals, self.st_slot, self.st_sign * g[self.st_elem])
+        if gv > 0:
+            np.add.at(vals, self.st_slot, self.st_sign * gv)
+        return vals
+
+    def _permute_rows(self, b):
+        bp = np.zeros(self.n)
+        bp[self.rowmap] = b
+        return bp
 
     def assemble(self, v, iph, gv):
         """Permuted CSC values and right-hand side of the companion system."""
         I, dIdV, _ = self.module(v, iph)
         f, g = -I, -dIdV
         ieq = f - g * v
-        vals = self.abase.copy()
-        np.add.at(vals, self.st_slot, self.st_sign * g[self.st_elem])
-        if gv > 0:
-            np.add.at(vals, self.st_slot, self.st_sign * gv)
         b = self.bbase.copy()
         pm = self.pos >= 0
         np.add.at(b, self.pos[pm], -ieq[pm])
         nm = self.neg >= 0
         np.add.at(b, self.neg[nm], ieq[nm])
-        bp = np.zeros(self.n)
-        bp[self.rowmap] = b
-        return vals, bp
+        return self._values(g, gv), self._permute_rows(b)
 
-    def iterate(self, v, iph, gv):
+    def residual(self, x, v, iph, gv, fg=None):
+        """b(v) - A(v) x of the companion system, unpermuted, evaluated branch
+        by branch: minus the sum of the branch currents leaving each node
+        (cables g (x_a - x_b); Norton buses G_N (x - 600); modules
+        f(v) + g (V(x) - v), plus g_virtual V(x) on homotopy stages)."""
+        if fg is None:
+            I, dIdV, _ = self.module(v, iph)
+            fg = (-I, -dIdV)
+        f, g = fg
+        r = np.zeros(self.n)
+        ic = self.cab_g * (x[self.cab_a] - x[self.cab_b])
+        np.add.at(r, self.cab_a, -ic)
+        np.add.at(r, self.cab_b, ic)
+        np.add.at(r, self.buses, -self.GN * (x[self.buses] - self.VN))
+        Ve = self.element_voltages(x)
+        ie = f + g * (Ve - v)
+        if gv > 0:
+            ie = ie + gv * Ve
+        pm = self.pos >= 0
+        np.add.at(r, self.pos[pm], -ie[pm])
+        nm = self.neg >= 0
+        np.add.at(r, self.neg[nm], ie[nm])
+        return r
+
+    def iterate(self, v, iph, gv, xprev=None):
+        """One Newton iteration from element voltages v (and, in the
+        correction form, the previous node voltages xprev; None = 0).
+        Returns (x, v_new, sum|dv|, max|dv|, LU values)."""
         I, dIdV, _ = self.module(v, iph)
         f, g = -I, -dIdV
-        ieq = f - g * v
-        vals = self.abase.copy()
-        np.add.at(vals, self.st_slot, self.st_sign * g[self.st_elem])
-        if gv > 0:
-            np.add.at(vals, self.st_slot, self.st_sign * gv)
-        b = self.bbase.copy()
-        pm = self.pos >= 0
-        np.add.at(b, self.pos[pm], -ieq[pm])
-        nm = self.neg >= 0
-        np.add.at(b, self.neg[nm], ieq[nm])
-        bp = np.zeros(self.n)
-        bp[self.rowmap] = b
+        vals = self._values(g, gv)
+        if self.form == "direct":
+            ieq = f - g * v
+            b = self.bbase.copy()
+            pm = self.pos >= 0
+            np.add.at(b, self.pos[pm], -ieq[pm])
+            nm = self.neg >= 0
+            np.add.at(b, self.neg[nm], ieq[nm])
+        else:
+            xprev = np.zeros(self.n) if xprev is None else xprev
+            b = self.residual(xprev, v, iph, gv, (f, g))
+        bp = self._permute_rows(b)
         st, bad = glu_csc_sequential(self.s, vals)
         if st != 0:
             raise RuntimeError(f"oracle GLU status {st} at column {bad}")
         substitute_lu_csc(self.s, vals, bp)
         x = bp[self.colmap]
-        vn = np.where(self.pos >= 0, x[np.maximum(self.pos, 0)], 0.0) - \
-            np.where(self.neg >= 0, x[np.maximum(self.neg, 0)], 0.0)
+        if self.form == "correction":
+            x = xprev + x
+        vn = self.element_voltages(x)
         st, v2, diff, mx = newton_update(v, vn)
         if st != 0:
             raise RuntimeError("oracle Newton: damping below 1e-13")
         return x, v2, diff, mx, vals
 
-    NOISE, STALL = 1e-8, 20   # last-stage noise-floor rule (as the library)
+    # last-stage noise-floor rule (as the library): 20 iterations with
+    # max |delta| at or below the form's FP64 floor
+    NOISE = {"direct": 1e-8, "correction": 1e-11}
+    STALL = 20
 
-    def solve(self, iph, v0, ladder=True, tol=1e-10, max_iters=5000):
+    def solve(self, iph, v0, ladder=True, tol=1e-10, max_iters=5000, x0=None):
         """Returns (x, v, iterations); self.last_max holds the final max |delta|
-        (> tol when the iteration stopped at the FP64 noise floor)."""
+        (> tol when the iteration stopped at the FP64 noise floor) and
+        self.stalled whether it did."""
         v = np.asarray(v0, float).copy()
         it = 0
         stages = LADDER if ladder else LADDER[-1:]
-        x = None
+        x = None if x0 is None else np.asarray(x0, float).copy()
+        self.stalled = False
         for k, gv in enumerate(stages):
             last = k == len(stages) - 1
             stall = 0
@@ -407,19 +477,20 @@ class PVNewton:
                 it += 1
                 if it > max_iters:
                     raise RuntimeError("oracle Newton did not converge")
-                x, v, diff, mx, lu = self.iterate(v, iph, gv)
+                x, v, diff, mx, lu = self.iterate(v, iph, gv, x)
                 if not last and diff < 1e-6 * self.n:
                     break
                 if last:
                     if mx <= tol:
                         break
-                    stall = stall + 1 if mx <= self.NOISE else 0
+                    stall = stall + 1 if mx <= self.NOISE[self.form] else 0
                     if stall >= self.STALL:
+                        self.stalled = True
                         break
         self.lu = lu
         self.last_max = mx
         return x, v, it
 
-    def power(self, x, buses):
-        V = x[np.asarray(buses)]
+    def power(self, x, buses=None):
+        V = x[self.buses if buses is None else np.asarray(buses)]
         return float(np.sum(V * self.GN * (V - self.VN)))

@@ -742,3 +742,11 @@ void orc_module_current(double V, double Iph, double I0, double a, double Rs,
   if (dIdIph)
     *dIdIph = -1.0 / dgdI;
 }
+
+void orc_module_current_many(int m, const double *V, const double *Iph,
+                             double I0, double a, double Rs, const double *Rsh,
+                             double *I, double *dIdV, double *dIdIph) {
+  for (int e = 0; e < m; e++)
+    orc_module_current(V[e], Iph[e], I0, a, Rs, Rsh[e], &I[e],
+                       dIdV ? &dIdV[e] : NULL, dIdIph ? &dIdIph[e] : NULL);
+}

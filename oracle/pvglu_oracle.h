@@ -125,6 +125,11 @@ int orc_remove_zeros_diagonal(int n, int *rp, int *ci, double *v, int *perm);
  * solved by safeguarded Newton on I. */
 void orc_module_current(double V, double Iph, double I0, double a, double Rs,
                         double Rsh, double *I, double *dIdV, double *dIdIph);
+/* orc_module_current over m modules (arrays of length m; dIdV / dIdIph may
+ * be NULL). */
+void orc_module_current_many(int m, const double *V, const double *Iph,
+                             double I0, double a, double Rs, const double *Rsh,
+                             double *I, double *dIdV, double *dIdIph);
 
 #ifdef __cplusplus
 }

@@ -25,6 +25,7 @@ LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libpvglu.so
 OK, E_SINGULAR, E_NODIAG, E_HIP, E_ARG, E_NOCONV, E_STATE = 0, -1, -2, -3, -4, -5, -6
 SCHED_AUTO, SCHED_LEVEL, SCHED_COMPONENT, SCHED_HYBRID = 0, 1, 2, 3
 CACHE_AUTO, CACHE_STREAMING, CACHE_DEFAULT = 0, 1, 2
+NEWTON_CORRECTION, NEWTON_DIRECT = 0, 1
 
 INFO = dict(n=0, nnz=1, nlev_lu=2, nchain_lu=3, nlev_l=4, nchain_l=5, nlev_u=6,
             nchain_u=7, nlev_lt=8, nchain_lt=9, nops=10, fused_lsolve=11, ncomp=12,
@@ -295,6 +296,8 @@ _PV_SIG = {
     "pvglu_pv_plan": (_VP, [_VP]),
     "pvglu_pv_newton_info": (C.c_int, [_VP, C.c_int, _VP, _VP]),
     "pvglu_pv_stamp": (C.c_int, [_VP, C.c_int, _VP, _VP, C.c_double, _VP, _VP, _VP]),
+    "pvglu_pv_set_newton_form": (C.c_int, [_VP, C.c_int]),
+    "pvglu_pv_get_newton_form": (C.c_int, [_VP]),
 }
 
 
@@ -425,6 +428,15 @@ class PV:
         _check(_pvlib().pvglu_pv_stamp(self._h, int(batch), _ptr(g_dev), _ptr(v_dev),
                                        float(g_virtual), _ptr(a_dev), _ptr(b_dev),
                                        _stream(stream)))
+
+    @property
+    def newton_form(self):
+        """NEWTON_CORRECTION (default) or NEWTON_DIRECT (the reference's step)."""
+        return _check(_pvlib().pvglu_pv_get_newton_form(self._h))
+
+    @newton_form.setter
+    def newton_form(self, form):
+        _check(_pvlib().pvglu_pv_set_newton_form(self._h, int(form)))
 
     def v0(self, batch=None):
         v = np.full(self.nmod, 600.0 / self.M)

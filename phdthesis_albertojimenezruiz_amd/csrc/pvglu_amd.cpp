@@ -1,5 +1,30 @@
 // pvglu_amd.cpp -- fill-reducing ordering and zero-diagonal row swaps.
 //
+// ---------------------------------------------------------------------------
+// Third-party notice.  pvglu_amd_order (everything in namespace amd below) is
+// a MODIFIED VERSION of AMD: a C++ translation of AMD 1.1's amd_aat.c,
+// amd_1.c, amd_2.c and amd_postorder.c that keeps AMD's algorithm, step
+// order, tie-breaks and many of its variable names (knt1/knt2, pme1/pme2,
+// elenme, nvpiv, degme, wnvi, dext, lemax, wflg/wbig).  Modifications: one
+// class over index vectors instead of one integer workspace, workspace growth
+// instead of garbage collection, the dense-row and postorder steps folded into
+// the driver, no Info/Control arrays.
+//
+//   AMD Version 1.1 (Jan. 21, 2004), Copyright (c) 2004 by Timothy A. Davis,
+//   Patrick R. Amestoy, and Iain S. Duff.
+//   Availability: http://www.cise.ufl.edu/research/sparse/amd
+//   Used by permission.
+//
+// AMD 1.1 is distributed under the AMD License of its README (the reference
+// vendors the sources, PV/AmdLib/amd.h:5-9, which point to that README, but
+// not the README itself).  That license permits use, copying, modification
+// and distribution of modified code provided that the Copyright, the License
+// and the Availability note are retained and a notice that the code was
+// modified is included; AMD is provided AS IS, WITH ABSOLUTELY NO WARRANTY
+// EXPRESSED OR IMPLIED, and any use is at the user's own risk.  This block
+// is that notice.
+// ---------------------------------------------------------------------------
+//
 // pvglu_amd_order restates the approximate minimum degree ordering of
 // Amestoy, Davis and Duff (ACM TOMS 30(3) 2004, Algorithm 837; SIAM J. Matrix
 // Anal. Appl. 17(4) 1996) as AMD 1.1 runs it with default controls (dense

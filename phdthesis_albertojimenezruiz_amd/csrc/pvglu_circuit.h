@@ -93,6 +93,14 @@ public:
   }
   void create_matrix_and_vector(double virtual_g, SparseMap &A,
                                 std::vector<double> &b) const;
+  // b - A x of create_matrix_and_vector's system at node voltages x
+  // (x[k - 1] for node k), evaluated branch by branch: the sources minus the
+  // currents leaving each node through admittances v (V1 - V2) and through
+  // nonlinear elements f(V) + f'(V) (Vx - V) [+ virtual_g Vx], V the
+  // element's linearization voltage and Vx its voltage at x.  The right-hand
+  // side of the correction form A d = b - A x (no reference counterpart).
+  void residual(double virtual_g, const std::vector<double> &x,
+                std::vector<double> &r) const;
   int nonlinear_count() const { return (int)nonlinear_.size(); }
   std::pair<int, int> nonlinear_nodes(int e) const {
     return {nonlinear_[e].n1, nonlinear_[e].n2};
@@ -147,6 +155,30 @@ inline void Circuit::create_matrix_and_vector(double virtual_g, SparseMap &A,
     stamp_i(b, e.n1, e.n2, -ieq);
     if (virtual_g > 0)
       stamp_g(A, e.n1, e.n2, virtual_g);
+  }
+}
+
+inline void Circuit::residual(double virtual_g, const std::vector<double> &x,
+                              std::vector<double> &r) const {
+  r.assign(nodes_, 0.0);
+  auto leave = [&](int n1, int n2, double i) {
+    if (n1)
+      r[n1 - 1] -= i;
+    if (n2)
+      r[n2 - 1] += i;
+  };
+  for (auto &e : admittances_)
+    leave(e.n1, e.n2, e.v * voltage_drop_dc(x, e.n1, e.n2));
+  for (auto &e : sources_)
+    stamp_i(r, e.n1, e.n2, e.v);
+  for (size_t k = 0; k < nonlinear_.size(); k++) {
+    const auto &e = nonlinear_[k];
+    const double V = v_iterate[k];
+    const double Vx = voltage_drop_dc(x, e.n1, e.n2);
+    double i = e.f(V) + e.df(V) * (Vx - V);
+    if (virtual_g > 0)
+      i = i + virtual_g * Vx;
+    leave(e.n1, e.n2, i);
   }
 }
 

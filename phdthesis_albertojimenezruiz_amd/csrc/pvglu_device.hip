@@ -9,6 +9,7 @@
 
 #include "pvglu_internal.h"
 #include "pvglu_launch.h"
+#include "pvglu_update.h"
 
 #include <algorithm>
 #include <chrono>
@@ -98,11 +99,20 @@ __global__ void k_fill_int(int *p, int n, int v) {
 __global__ void __launch_bounds__(256)
     k_newton_update(int ne, int B, const int *__restrict__ pos,
                     const int *__restrict__ neg, const double *__restrict__ x,
-                    double *__restrict__ v, double *delta_sum,
+                    double *__restrict__ v, unsigned long long *delta_sum,
                     unsigned long long *delta_max, const int *active,
                     int *status) {
-  update_elements(ne, B, pos, neg, x, v, delta_sum, delta_max, active, status,
-                  [](int, int, double) {});
+  update_elements(ne, B, pos, neg, x, nullptr, v, delta_sum, delta_max, active,
+                  status, [](int, int, double, double) {});
+}
+
+// The fixed-point sums of k_newton_update converted to volts in place.
+__global__ void k_fixed_to_delta(int B, double *delta_sum) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < B) {
+    unsigned long long *u = (unsigned long long *)delta_sum;
+    delta_sum[k] = fixed_to_delta(u[k]);
+  }
 }
 
 // -------------------------------------------------------------- host side
@@ -724,8 +734,10 @@ extern "C" int pvglu_newton_update(pvglu_plan *P, int ne, int batch,
   hipLaunchKernelGGL(k_newton_update,
                      dim3(chunks, (batch + (1 << lt) - 1) >> lt), dim3(256), 0,
                      (hipStream_t)stream, ne, batch, pos, neg, x, v_elem,
-                     delta_sum, (unsigned long long *)delta_max, active,
-                     P->dev->status);
+                     (unsigned long long *)delta_sum,
+                     (unsigned long long *)delta_max, active, P->dev->status);
+  hipLaunchKernelGGL(k_fixed_to_delta, dim3((batch + 255) / 256), dim3(256), 0,
+                     (hipStream_t)stream, batch, delta_sum);
   HIP_TRY(hipGetLastError());
   return PVGLU_OK;
 }

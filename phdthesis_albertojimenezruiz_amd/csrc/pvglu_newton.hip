@@ -18,6 +18,7 @@
 #include "pvglu_launch.h"
 #include "pvglu_module.h"
 #include "pvglu_pv.h"
+#include "pvglu_update.h"
 
 #include <algorithm>
 #include <chrono>
@@ -65,19 +66,32 @@ struct NewtonDev {
       *b_mod = nullptr, *b_sgn = nullptr, *pos = nullptr, *neg = nullptr,
       *rpos = nullptr, *rneg = nullptr, *cab_ra = nullptr, *cab_rb = nullptr,
       *cab_ca = nullptr, *cab_cb = nullptr, *bus_c = nullptr;
+  // linear branches per permuted row (correction-form residual): terminals
+  // ca, cb (permuted unknowns; cb < 0: a source of EMF l_e), signed
+  // conductance l_g (+ at the row of ca, - at the row of cb)
+  int *l_ptr = nullptr, *l_ca = nullptr, *l_cb = nullptr;
+  double *l_g = nullptr, *l_e = nullptr;
   // instance-side
+  // ieq: the companion current source f - G v (direct form) or the branch
+  // current at the new node voltages f + G (V - v) (correction form)
   double *G = nullptr, *v = nullptr, *geq = nullptr, *ieq = nullptr,
          *A = nullptr, *LU = nullptr, *b = nullptr, *x = nullptr,
-         *dsum = nullptr, *dmax = nullptr, *gvs = nullptr, *gP = nullptr,
+         *X[2] = {nullptr, nullptr}, *dx = nullptr,
+         *gvs = nullptr, *gP = nullptr,
          *mu = nullptr, *dG = nullptr, *dgc = nullptr, *lastmax = nullptr,
          *ladder = nullptr;
+  // per-instance sum (2^-44 V fixed point) and max (double bits) of |change|
+  unsigned long long *dsum = nullptr, *dmax = nullptr;
   int *stage = nullptr, *active = nullptr, *stall = nullptr, *iters = nullptr,
       *remaining = nullptr;
   int nstages = 12; // stages of the current solve (12 = ladder, 1 = last only)
   double tol = 1e-10;    // captured into the graph; re-captured when changed
+  int form = PVGLU_NEWTON_CORRECTION; // form the graphs were captured for
   hipStream_t s = nullptr;
-  hipGraphExec_t exec = nullptr;
-  hipGraph_t graph = nullptr;
+  // one graph per iteration; the correction form alternates two (parity p
+  // reads node voltages X[p] and writes X[1 - p])
+  hipGraphExec_t exec = nullptr, exec2 = nullptr;
+  hipGraph_t graph = nullptr, graph2 = nullptr;
   std::vector<void *> allocs;
   std::vector<void *> inst_allocs;
 };
@@ -85,10 +99,12 @@ struct NewtonDev {
 void newton_dev_destroy(NewtonDev *d) {
   if (!d)
     return;
-  if (d->exec)
-    (void)hipGraphExecDestroy(d->exec);
-  if (d->graph)
-    (void)hipGraphDestroy(d->graph);
+  for (hipGraphExec_t e : {d->exec, d->exec2})
+    if (e)
+      (void)hipGraphExecDestroy(e);
+  for (hipGraph_t g : {d->graph, d->graph2})
+    if (g)
+      (void)hipGraphDestroy(g);
   if (d->s)
     (void)hipStreamDestroy(d->s);
   for (void *p : d->allocs)
@@ -101,6 +117,9 @@ void newton_dev_destroy(NewtonDev *d) {
 } // namespace pvglu
 
 // ----------------------------------------------------------------- kernels
+
+// Rows per block of the row gathers (stamping, correction-form residual).
+constexpr int kStampRows = 8;
 
 // Companion model of each module (PV/002b_sistema_electrico_base_matriz.cpp:183-203):
 // element current f(V) = -I(V) flows + -> -, G = f'(V), I_eq = f(V) - G V.
@@ -129,11 +148,17 @@ __global__ void k_eval(int nm, int B, const double *__restrict__ Girr,
 // control kernel freezes right after this update gets an evaluation that
 // nothing reads (stamping skips frozen instances); the control kernel does
 // not change voltages.
+// Direct form (dx null): x holds the new node voltages; ieq <- f - G v.
+// Correction form: x + dx are the new node voltages (x = the previous ones,
+// dx = the solve's correction) and ieq <- f + G (V - v), the linearized
+// branch current at those node voltages, which the next iteration's residual
+// gathers (V - v = 0 exactly when the step was not damped).
 __global__ void __launch_bounds__(256)
     k_update_eval(int ne, int B, const int *__restrict__ pos,
                   const int *__restrict__ neg, const double *__restrict__ x,
-                  double *__restrict__ v, double *dsum,
-                  unsigned long long *dmax, const int *active, int *status,
+                  const double *__restrict__ dx, double *__restrict__ v,
+                  unsigned long long *dsum, unsigned long long *dmax,
+                  const int *active, int *status,
                   const double *__restrict__ Girr,
                   const double *__restrict__ rsh,
                   const double *__restrict__ lk, double *__restrict__ geq,
@@ -141,15 +166,134 @@ __global__ void __launch_bounds__(256)
   // the control kernel after this one counts the unfinished instances
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
     *remaining = 0;
-  update_elements(ne, B, pos, neg, x, v, dsum, dmax, active, status,
-                  [&](int e, int in, double V) {
+  update_elements(ne, B, pos, neg, x, dx, v, dsum, dmax, active, status,
+                  [&](int e, int in, double V, double Vx) {
                     const size_t gid = (size_t)e * B + in;
                     const ModuleEval me = module_current_lk(
                         V, kIphPerIrradiance * Girr[gid], rsh[e], lk[e]);
                     const double f = -me.I, g = -me.dIdV;
                     geq[gid] = g;
-                    ieq[gid] = f - g * V;
+                    ieq[gid] = dx ? f + g * (Vx - V) : f - g * V;
                   });
+}
+
+// Correction-form right-hand side.  Per permuted row (equation of one node)
+// and instance: minus the sum of the currents leaving the node through its
+// branches, at node voltages x = xin + dx (the previous iteration's solution
+// plus its correction):
+//   modules  ibr (from k_update_eval), + g_virtual (x[pos] - x[neg]) on
+//            homotopy stages;
+//   linear   l_g (x[ca] - x[cb]), cb < 0 meaning a source of EMF l_e (the
+//            Norton bus: G_N (x - 600)).
+// Each difference of node voltages is formed before it is scaled, so a
+// string's ~600 V node voltages cancel exactly and the residual carries the
+// rounding of the branch currents only.  The kernel also writes
+// xout = xin + dx for the column of the same index (every unknown once), the
+// node voltages the solve's correction applies to.  Frozen instances get a
+// zero right-hand side, so their correction is zero and xout stays put
+// after the first replay that carries their last correction over.
+// Layout: one block per kStampRows rows x up to 256 instances, as the
+// stamping gathers.
+__global__ void __launch_bounds__(256)
+    k_residual(int n, int B, const int *__restrict__ mptr,
+               const int *__restrict__ mmod, const int *__restrict__ msgn,
+               const int *__restrict__ lptr, const int *__restrict__ lca,
+               const int *__restrict__ lcb, const double *__restrict__ lg,
+               const double *__restrict__ le, const int *__restrict__ pos,
+               const int *__restrict__ neg, const double *__restrict__ ibr,
+               const double *__restrict__ gvp, const double *__restrict__ xin,
+               const double *__restrict__ dx, double *__restrict__ xout,
+               double *__restrict__ r, const int *active) {
+  const int inst = (int)(blockIdx.y * blockDim.x + threadIdx.x);
+  if (inst >= B)
+    return;
+  const int r0 = (int)blockIdx.x * kStampRows;
+  const bool live = !active || active[inst];
+  const double gv = live ? gvp[inst] : -1.0;
+  auto node = [&](int k) {
+    if (k < 0)
+      return 0.0;
+    const size_t i = (size_t)k * B + inst;
+    return xin[i] + dx[i];
+  };
+  int mp[kStampRows + 1], lp[kStampRows + 1];
+#pragma unroll
+  for (int q = 0; q <= kStampRows; ++q) {
+    mp[q] = mptr[min(r0 + q, n)];
+    lp[q] = lptr[min(r0 + q, n)];
+  }
+  bool simple = gv <= 0;
+#pragma unroll
+  for (int q = 0; q < kStampRows; ++q)
+    simple = simple && mp[q + 1] - mp[q] <= 2 && lp[q + 1] - lp[q] <= 1;
+  if (simple) {
+    // every load first (absent terms load a valid stand-in), then combine
+    double xo[kStampRows], i0[kStampRows], i1[kStampRows], xa[kStampRows],
+        xb[kStampRows];
+#pragma unroll
+    for (int q = 0; q < kStampRows; ++q) {
+      const int row = min(r0 + q, n - 1);
+      xo[q] = node(row);
+      const int cm = mp[q + 1] - mp[q];
+      const int m0 = cm > 0 ? mmod[mp[q]] : 0;
+      const int m1 = cm > 1 ? mmod[mp[q] + 1] : m0;
+      i0[q] = ibr[(size_t)m0 * B + inst];
+      i1[q] = ibr[(size_t)m1 * B + inst];
+      const bool hl = lp[q + 1] > lp[q];
+      xa[q] = node(hl ? lca[lp[q]] : row);
+      const int cb = hl ? lcb[lp[q]] : row;
+      xb[q] = cb >= 0 ? node(cb) : le[lp[q]];
+    }
+#pragma unroll
+    for (int q = 0; q < kStampRows; ++q) {
+      const int row = r0 + q;
+      if (row >= n)
+        break;
+      const size_t gi = (size_t)row * B + inst;
+      xout[gi] = xo[q];
+      double acc = 0.0;
+      const int cm = mp[q + 1] - mp[q];
+      if (live) {
+        if (cm > 0)
+          acc = acc - (double)msgn[mp[q]] * i0[q];
+        if (cm > 1)
+          acc = acc - (double)msgn[mp[q] + 1] * i1[q];
+        if (lp[q + 1] > lp[q])
+          acc = acc - lg[lp[q]] * (xa[q] - xb[q]);
+      }
+      r[gi] = acc;
+    }
+    return;
+  }
+  for (int q = 0; q < kStampRows && r0 + q < n; ++q) {
+    const int row = r0 + q;
+    const size_t gi = (size_t)row * B + inst;
+    xout[gi] = node(row);
+    double acc = 0.0;
+    if (live) {
+      for (int k = mp[q]; k < mp[q + 1]; ++k) {
+        const int e = mmod[k];
+        double ie = ibr[(size_t)e * B + inst];
+        if (gv > 0)
+          ie = ie + gv * (node(pos[e]) - node(neg[e]));
+        acc = acc - (double)msgn[k] * ie;
+      }
+      for (int k = lp[q]; k < lp[q + 1]; ++k) {
+        const int cb = lcb[k];
+        const double xb = cb >= 0 ? node(cb) : le[k];
+        acc = acc - lg[k] * (node(lca[k]) - xb);
+      }
+    }
+    r[gi] = acc;
+  }
+}
+
+// x = a + b elementwise (the correction form's final node voltages)
+__global__ void k_add(long count, const double *__restrict__ a,
+                      const double *__restrict__ b, double *__restrict__ x) {
+  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid < count)
+    x[gid] = a[gid] + b[gid];
 }
 
 // Stamping gathers.  A block covers kStampRows consecutive rows (slots of A,
@@ -163,7 +307,6 @@ __global__ void __launch_bounds__(256)
 // row, so a block is one memory round trip; otherwise the rows are combined
 // one at a time.  The
 // operation order per row is fixed (contributions in map order).
-constexpr int kStampRows = 8;
 
 inline RowGrid stamp_grid(long rows, int B) {
   const int bs = B >= 256 ? 256 : ((B + 63) / 64) * 64;
@@ -330,12 +473,13 @@ __global__ void k_fill_double(double *p, int n, double v) {
 // Per-instance Newton state at the start of a solve.
 __global__ void k_newton_init(int B, double gv0, int *stage, double *gvs,
                               int *active, int *stall, int *iters,
-                              double *lastmax, double *dsum, double *dmax) {
+                              double *lastmax, unsigned long long *dsum,
+                              unsigned long long *dmax) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= B)
     return;
-  dsum[k] = 0.0; // the update accumulates into these; control clears them
-  dmax[k] = 0.0;
+  dsum[k] = 0; // the update accumulates into these; control clears them
+  dmax[k] = 0;
   stage[k] = 0;
   gvs[k] = gv0;
   active[k] = 1;
@@ -353,8 +497,8 @@ __global__ void k_newton_init(int B, double gv0, int *stage, double *gvs,
 __global__ void k_newton_control(int B, int nstages,
                                  const double *__restrict__ ladder,
                                  double stage_crit, double tol, double noise,
-                                 int stall_max, double *dsum, double *dmax,
-                                 int *stage,
+                                 int stall_max, unsigned long long *dsum,
+                                 unsigned long long *dmax, int *stage,
                                  double *gvs, int *active, int *stall,
                                  int *iters, double *lastmax, int *remaining) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -362,10 +506,10 @@ __global__ void k_newton_control(int B, int nstages,
     return;
   iters[k]++;
   const int st = stage[k];
-  const double m = dmax[k];
-  const double sum = dsum[k];
-  dsum[k] = 0.0; // cleared for the next iteration's update
-  dmax[k] = 0.0;
+  const double m = __longlong_as_double((long long)dmax[k]);
+  const double sum = fixed_to_delta(dsum[k]);
+  dsum[k] = 0; // cleared for the next iteration's update
+  dmax[k] = 0;
   lastmax[k] = m;
   if (st < nstages - 1) {
     if (sum < stage_crit) {
@@ -509,6 +653,31 @@ int newton_setup(pvglu_pv *pv) {
   }
   for (int k = 0; k < pv->I; k++)
     busc[k] = colmap[pv->bus[k]];
+  // linear branches per permuted row, in a fixed order: cables (both ends),
+  // then the Norton source of a bus
+  struct LinTerm {
+    int ca, cb;
+    double g, e;
+  };
+  std::vector<std::vector<LinTerm>> lin(n);
+  for (int c = 0; c < pv->ncab; c++) {
+    const int a = pv->cab_a[c], b = pv->cab_b[c];
+    lin[rowmap[a]].push_back({colmap[a], colmap[b], pv->cab_g[c], 0.0});
+    lin[rowmap[b]].push_back({colmap[a], colmap[b], -pv->cab_g[c], 0.0});
+  }
+  for (int bidx : pv->bus)
+    lin[rowmap[bidx]].push_back({colmap[bidx], -1, kNortonG, kNortonV});
+  std::vector<int> l_ptr(1, 0), l_ca, l_cb;
+  std::vector<double> l_g, l_e;
+  for (auto &row : lin) {
+    for (auto &t : row) {
+      l_ca.push_back(t.ca);
+      l_cb.push_back(t.cb);
+      l_g.push_back(t.g);
+      l_e.push_back(t.e);
+    }
+    l_ptr.push_back((int)l_ca.size());
+  }
   auto *d = new NewtonDev();
   pv->nd = d;
   auto &A = d->allocs;
@@ -521,7 +690,9 @@ int newton_setup(pvglu_pv *pv) {
       (st = dupload(A, &d->rpos, rpos)) || (st = dupload(A, &d->rneg, rneg)) ||
       (st = dupload(A, &d->cab_ra, cra)) || (st = dupload(A, &d->cab_rb, crb)) ||
       (st = dupload(A, &d->cab_ca, cca)) || (st = dupload(A, &d->cab_cb, ccb)) ||
-      (st = dupload(A, &d->bus_c, busc)) ||
+      (st = dupload(A, &d->bus_c, busc)) || (st = dupload(A, &d->l_ptr, l_ptr)) ||
+      (st = dupload(A, &d->l_ca, l_ca)) || (st = dupload(A, &d->l_cb, l_cb)) ||
+      (st = dupload(A, &d->l_g, l_g)) || (st = dupload(A, &d->l_e, l_e)) ||
       (st = dupload(A, &d->ladder,
                     std::vector<double>(kLadder, kLadder + 12))) ||
       (st = dupload(A, &d->lk, std::vector<double>(pv->nmod, 0.0))))
@@ -536,18 +707,24 @@ int newton_setup(pvglu_pv *pv) {
   return PVGLU_OK;
 }
 
+void drop_graphs(NewtonDev *d) {
+  for (hipGraphExec_t *e : {&d->exec, &d->exec2})
+    if (*e) {
+      (void)hipGraphExecDestroy(*e);
+      *e = nullptr;
+    }
+  for (hipGraph_t *g : {&d->graph, &d->graph2})
+    if (*g) {
+      (void)hipGraphDestroy(*g);
+      *g = nullptr;
+    }
+}
+
 int newton_alloc_instances(pvglu_pv *pv, int B) {
   NewtonDev *d = pv->nd;
   if (d->B == B)
     return PVGLU_OK;
-  if (d->exec) {
-    (void)hipGraphExecDestroy(d->exec);
-    d->exec = nullptr;
-  }
-  if (d->graph) {
-    (void)hipGraphDestroy(d->graph);
-    d->graph = nullptr;
-  }
+  drop_graphs(d);
   for (void *p : d->inst_allocs)
     (void)hipFree(p);
   d->inst_allocs.clear();
@@ -559,6 +736,8 @@ int newton_alloc_instances(pvglu_pv *pv, int B) {
       (st = dalloc(I, &d->geq, nm * B)) || (st = dalloc(I, &d->ieq, nm * B)) ||
       (st = dalloc(I, &d->A, nnz * B)) || (st = dalloc(I, &d->LU, nnz * B)) ||
       (st = dalloc(I, &d->b, n * B)) || (st = dalloc(I, &d->x, n * B)) ||
+      (st = dalloc(I, &d->X[0], n * B)) || (st = dalloc(I, &d->X[1], n * B)) ||
+      (st = dalloc(I, &d->dx, n * B)) ||
       (st = dalloc(I, &d->dsum, (size_t)B)) || (st = dalloc(I, &d->dmax, (size_t)B)) ||
       (st = dalloc(I, &d->gP, n * B)) || (st = dalloc(I, &d->mu, n * B)) ||
       (st = dalloc(I, &d->dG, nm * B)) || (st = dalloc(I, &d->dgc, nc * B)) ||
@@ -597,19 +776,51 @@ int launch_stamp(pvglu_pv *pv, int B, const int *active, bool eval) {
   return PVGLU_OK;
 }
 
-// Stall rule of the last stage: max |delta| <= kNoise for kStallMax iterations.
-constexpr double kNoise = 1e-8;
+// Stall rule of the last stage: max |delta| <= noise for kStallMax
+// iterations.  The noise level is the FP64 floor of each form: the direct
+// form's iterates carry the solve's rounding of the ~600 V node voltages
+// (up to ~1e-9 V behind weak modules), the correction form's only the
+// rounding of the branch currents (~1e-12 V).
+constexpr double kNoiseDirect = 1e-8;
+constexpr double kNoiseCorrection = 1e-11;
 constexpr int kStallMax = 20;
+inline double noise_of(int form) {
+  return form == PVGLU_NEWTON_DIRECT ? kNoiseDirect : kNoiseCorrection;
+}
 
-int launch_iteration(pvglu_pv *pv, int B) {
+// One Newton iteration on d->s.  Direct form: stamping of A and b, solve for
+// the node voltages x.  Correction form (parity p): stamping of A, the
+// branch-current residual at X[p] + dx (writing those node voltages to
+// X[1 - p]), solve for the correction dx.  Then the fused update +
+// evaluation and the per-instance control.
+int launch_iteration(pvglu_pv *pv, int B, int parity) {
   NewtonDev *d = pv->nd;
+  const bool corr = d->form == PVGLU_NEWTON_CORRECTION;
+  const int n = pv->n, nnz = pv->plan->nnz;
   // the companion models of this iteration come from the previous
   // iteration's fused update (or the evaluation before the first replay)
-  int st = launch_stamp(pv, B, d->active, false);
-  if (st)
-    return st;
-  if ((st = pvglu_refactor_solve(pv->plan, d->A, d->LU, d->b, d->x, B, d->s)))
-    return st;
+  int st;
+  const double *xnew = d->x, *dx = nullptr;
+  if (!corr) {
+    if ((st = launch_stamp(pv, B, d->active, false)))
+      return st;
+    if ((st = pvglu_refactor_solve(pv->plan, d->A, d->LU, d->b, d->x, B, d->s)))
+      return st;
+  } else {
+    const RowGrid ga = stamp_grid(nnz, B), gb = stamp_grid(n, B);
+    hipLaunchKernelGGL(k_stamp_a, ga.grid, ga.block, 0, d->s, nnz, B,
+                       d->base_a, d->a_ptr, d->a_mod, d->a_sgn, d->geq, d->gvs,
+                       d->A, d->active);
+    hipLaunchKernelGGL(k_residual, gb.grid, gb.block, 0, d->s, n, B, d->b_ptr,
+                       d->b_mod, d->b_sgn, d->l_ptr, d->l_ca, d->l_cb, d->l_g,
+                       d->l_e, d->pos, d->neg, d->ieq, d->gvs, d->X[parity],
+                       d->dx, d->X[1 - parity], d->b, d->active);
+    HIP_TRY(hipGetLastError());
+    if ((st = pvglu_refactor_solve(pv->plan, d->A, d->LU, d->b, d->dx, B, d->s)))
+      return st;
+    xnew = d->X[1 - parity];
+    dx = d->dx;
+  }
   int *status = device_status(pv->plan);
   if (!status)
     return fail(PVGLU_E_STATE, "batched Newton: plan has no device state");
@@ -619,24 +830,24 @@ int launch_iteration(pvglu_pv *pv, int B) {
   hipLaunchKernelGGL(k_update_eval,
                      dim3((nm + per_block - 1) / per_block,
                           (B + (1 << lt) - 1) >> lt),
-                     dim3(256), 0, d->s, nm, B, d->pos, d->neg, d->x, d->v,
-                     d->dsum, (unsigned long long *)d->dmax, d->active, status,
-                     d->G, d->rsh, d->lk, d->geq, d->ieq, d->remaining);
+                     dim3(256), 0, d->s, nm, B, d->pos, d->neg, xnew, dx, d->v,
+                     d->dsum, d->dmax, d->active, status, d->G, d->rsh, d->lk,
+                     d->geq, d->ieq, d->remaining);
   const double *ladder = d->ladder + (12 - d->nstages);
   hipLaunchKernelGGL(k_newton_control, grid_for(B), dim3(256), 0, d->s, B,
-                     d->nstages, ladder, 1e-6 * pv->n, d->tol, kNoise, kStallMax,
-                     d->dsum, d->dmax, d->stage, d->gvs, d->active, d->stall,
-                     d->iters, d->lastmax, d->remaining);
+                     d->nstages, ladder, 1e-6 * pv->n, d->tol,
+                     noise_of(d->form), kStallMax, d->dsum, d->dmax, d->stage,
+                     d->gvs, d->active, d->stall, d->iters, d->lastmax,
+                     d->remaining);
   HIP_TRY(hipGetLastError());
   return PVGLU_OK;
 }
 
-int build_graph(pvglu_pv *pv, int B) {
+int capture_one(pvglu_pv *pv, int B, int parity, hipGraph_t *graph,
+                hipGraphExec_t *exec) {
   NewtonDev *d = pv->nd;
-  if (d->exec)
-    return PVGLU_OK;
   HIP_TRY(hipStreamBeginCapture(d->s, hipStreamCaptureModeThreadLocal));
-  int st = launch_iteration(pv, B);
+  int st = launch_iteration(pv, B, parity);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(d->s, &g);
   if (st) {
@@ -646,9 +857,21 @@ int build_graph(pvglu_pv *pv, int B) {
   }
   if (e != hipSuccess)
     return fail(PVGLU_E_HIP, "graph capture: %s", hipGetErrorString(e));
-  d->graph = g;
-  HIP_TRY(hipGraphInstantiate(&d->exec, g, nullptr, nullptr, 0));
+  *graph = g;
+  HIP_TRY(hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
   return PVGLU_OK;
+}
+
+int build_graph(pvglu_pv *pv, int B) {
+  NewtonDev *d = pv->nd;
+  if (d->exec)
+    return PVGLU_OK;
+  int st = capture_one(pv, B, 0, &d->graph, &d->exec);
+  if (!st && d->form == PVGLU_NEWTON_CORRECTION)
+    st = capture_one(pv, B, 1, &d->graph2, &d->exec2);
+  if (st)
+    drop_graphs(d);
+  return st;
 }
 
 } // namespace
@@ -676,29 +899,36 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
   if ((st = pvglu_status_reset(pv->plan, d->s)))
     return st;
   const int nst = ladder ? 12 : 1;
-  if (d->exec && (d->nstages != nst || d->tol != tol)) {
-    (void)hipGraphExecDestroy(d->exec);
-    (void)hipGraphDestroy(d->graph);
-    d->exec = nullptr;
-    d->graph = nullptr;
-  }
+  if (d->exec && (d->nstages != nst || d->tol != tol || d->form != pv->form))
+    drop_graphs(d);
   d->nstages = nst;
   d->tol = tol;
+  d->form = pv->form;
+  const bool corr = d->form == PVGLU_NEWTON_CORRECTION;
   hipLaunchKernelGGL(k_newton_init, grid_for(B), dim3(256), 0, d->s, B,
                      kLadder[12 - nst], d->stage, d->gvs, d->active, d->stall,
                      d->iters, d->lastmax, d->dsum, d->dmax);
+  if (corr) {
+    // node voltages start at 0 with a zero correction: the first
+    // iteration's right-hand side is the companion b (its x = A^-1 b)
+    const long cnt = (long)pv->n * B;
+    hipLaunchKernelGGL(k_zero, grid_for(cnt), dim3(256), 0, d->s, cnt, d->X[0]);
+    hipLaunchKernelGGL(k_zero, grid_for(cnt), dim3(256), 0, d->s, cnt, d->dx);
+  }
   launch_eval(pv, B, d->active); // the first replay's companion models
   HIP_TRY(hipGetLastError());
   if ((st = build_graph(pv, B)))
     return st;
   // The host polls the remaining-instance counter after every kPoll graph
   // replays: a replay after an instance has converged leaves it unchanged
-  // (frozen: no evaluation, stamping or update, and the refactor + solve of
-  // its unchanged system reproduces its solution bit for bit), so the
-  // extra replays of a burst cost time only, and the GPU no longer idles for
-  // a host round trip after each iteration.
+  // (frozen: no evaluation, stamping or update; in the direct form the
+  // refactor + solve of its unchanged system reproduces its solution bit for
+  // bit, in the correction form its right-hand side and so its correction
+  // are zero), so the extra replays of a burst cost time only, and the GPU
+  // no longer idles for a host round trip after each iteration.  The
+  // correction form alternates its two graphs (replay j reads X[j % 2]).
   constexpr int kPoll = 4;
-  int iters = 0, remaining = 1;
+  int iters = 0, remaining = 1, parity = 0;
   auto t0 = std::chrono::steady_clock::now();
   while (remaining > 0) {
     if (iters >= max_iters) {
@@ -709,11 +939,23 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
                   iters, remaining);
     }
     const int burst = std::min(kPoll, max_iters - iters);
-    for (int k = 0; k < burst; ++k)
-      HIP_TRY(hipGraphLaunch(d->exec, d->s));
+    for (int k = 0; k < burst; ++k) {
+      HIP_TRY(hipGraphLaunch(parity ? d->exec2 : d->exec, d->s));
+      if (corr)
+        parity ^= 1;
+    }
     iters += burst;
     HIP_TRY(hipMemcpyAsync(&remaining, d->remaining, sizeof(int),
                            hipMemcpyDeviceToHost, d->s));
+    HIP_TRY(hipStreamSynchronize(d->s));
+  }
+  if (corr) {
+    // the last replay wrote X[parity]; its correction is still in dx (zero
+    // for the instances frozen before it)
+    const long cnt = (long)pv->n * B;
+    hipLaunchKernelGGL(k_add, grid_for(cnt), dim3(256), 0, d->s, cnt,
+                       d->X[parity], d->dx, d->x);
+    HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(d->s));
   }
   auto t1 = std::chrono::steady_clock::now();
@@ -792,10 +1034,11 @@ extern "C" int pvglu_pv_gradients(pvglu_pv *pv, int batch, const double *g,
   NewtonDev *d = pv->nd;
   const int B = batch, n = pv->n, nm = pv->nmod, nc = pv->ncab;
   // Jacobian at the converged element voltages: stamp + refactor (the
-  // companion matrix at V_e = V_branch(x) is dF/dx), then the last solve's x
+  // companion matrix at V_e = V_branch(x) is dF/dx); x is the Newton's
+  // converged node voltages
   if ((st = launch_stamp(pv, B, nullptr, true)))
     return st;
-  if ((st = pvglu_refactor_solve(pv->plan, d->A, d->LU, d->b, d->x, B, d->s)))
+  if ((st = pvglu_refactor(pv->plan, d->A, d->LU, B, d->s)))
     return st;
   hipLaunchKernelGGL(k_zero, grid_for((long)n * B), dim3(256), 0, d->s,
                      (long)n * B, d->gP);
@@ -867,7 +1110,10 @@ extern "C" int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g,
   HIP_TRY(hipMemcpy(d->v, v_module, sizeof(double) * nm, hipMemcpyHostToDevice));
   if ((st = pvglu_status_reset(P, nullptr)))
     return st;
-  std::vector<double> vals(nnz), bvec, tmp(n);
+  const bool corr = pv->form == PVGLU_NEWTON_CORRECTION;
+  // correction form: node voltages of the last iteration (unpermuted; 0 at
+  // the start, so the first right-hand side is b) and their permuted copy
+  std::vector<double> vals(nnz), bvec, tmp(n), xprev(n, 0.0), xcol(n);
   double t_host = 0, t_dev = 0;
   auto T0 = std::chrono::steady_clock::now();
   int it = 0;
@@ -891,6 +1137,8 @@ extern "C" int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g,
       // create_matrix_and_vector + permutations + CSC value fill (000:185-210)
       SparseMap A;
       pv->circuit.create_matrix_and_vector(stages[si], A, bvec);
+      if (corr)
+        pv->circuit.residual(stages[si], xprev, bvec);
       A = A.interchange_row_columns(pv->Pinv.data());
       A = A.exchange_rows(pv->Prow.data());
       for (int i = 0; i < n; i++)
@@ -904,8 +1152,20 @@ extern "C" int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g,
       HIP_TRY(hipMemcpy(d->a, vals.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(d->b, bvec.data(), sizeof(double) * n, hipMemcpyHostToDevice));
       if ((st = pvglu_refactor(P, d->a, d->lu, 1, nullptr)) ||
-          (st = pvglu_solve(P, d->lu, d->b, d->x, 1, nullptr)) ||
-          (st = pvglu_newton_update(P, nm, 1, d->pos, d->neg, d->x, d->v, d->dsum,
+          (st = pvglu_solve(P, d->lu, d->b, d->x, 1, nullptr)))
+        return st;
+      if (corr) {
+        // x = x_prev + d, in unpermuted and permuted order
+        HIP_TRY(hipMemcpy(tmp.data(), d->x, sizeof(double) * n,
+                          hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; i++) {
+          xprev[i] = xprev[i] + tmp[pv->Pinv[i]];
+          xcol[pv->Pinv[i]] = xprev[i];
+        }
+        HIP_TRY(hipMemcpy(d->x, xcol.data(), sizeof(double) * n,
+                          hipMemcpyHostToDevice));
+      }
+      if ((st = pvglu_newton_update(P, nm, 1, d->pos, d->neg, d->x, d->v, d->dsum,
                                     d->dmax, nullptr, nullptr)))
         return st;
       double dsum = 0, dmax = 0;
@@ -922,7 +1182,7 @@ extern "C" int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g,
       if (fixed_iters <= 0 && last) {
         if (dmax <= tol)
           break;
-        stall = dmax <= kNoise ? stall + 1 : 0; // FP64 noise floor
+        stall = dmax <= noise_of(pv->form) ? stall + 1 : 0; // FP64 noise floor
         if (stall >= kStallMax)
           break;
       }
@@ -997,6 +1257,20 @@ extern "C" int pvglu_pv_stamp(pvglu_pv *pv, int batch, const double *g,
                      d->b_ptr, d->b_mod, d->b_sgn, ieq, b, nullptr);
   HIP_TRY(hipGetLastError());
   return PVGLU_OK;
+}
+
+extern "C" int pvglu_pv_set_newton_form(pvglu_pv *pv, int form) {
+  clear_error();
+  if (!pv || (form != PVGLU_NEWTON_CORRECTION && form != PVGLU_NEWTON_DIRECT))
+    return fail(PVGLU_E_ARG, "set_newton_form: bad arguments");
+  pv->form = form;
+  return PVGLU_OK;
+}
+
+extern "C" int pvglu_pv_get_newton_form(const pvglu_pv *pv) {
+  if (!pv)
+    return fail(PVGLU_E_ARG, "get_newton_form: null");
+  return pv->form;
 }
 
 extern "C" int pvglu_pv_newton_info(const pvglu_pv *pv, int batch, int *iters,

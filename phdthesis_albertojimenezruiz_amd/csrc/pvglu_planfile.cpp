@@ -185,6 +185,31 @@ extern "C" int pvglu_plan_load(const char *path, pvglu_plan **out, int *P,
   if ((!fileP.empty() && (int)fileP.size() != n) ||
       (!fileProw.empty() && (int)fileProw.size() != n))
     return fail(PVGLU_E_ARG, "plan_load: ordering sections of wrong length");
+  // The checksum only catches accidental corruption: the ordering's values
+  // are checked too.  P must be a permutation of [0, n); Prow a permutation
+  // that is its own inverse, as the Newton loop's exchange_rows(Prow) only
+  // undoes the recorded row swaps then (PV/000_PVPP_simulator.cpp:195-196).
+  auto is_perm = [n](const std::vector<int> &p) {
+    std::vector<char> seen((size_t)n, 0);
+    for (int v : p) {
+      if (v < 0 || v >= n || seen[(size_t)v])
+        return false;
+      seen[(size_t)v] = 1;
+    }
+    return true;
+  };
+  if (!fileP.empty() && !is_perm(fileP))
+    return fail(PVGLU_E_ARG, "plan_load: stored P of %s is not a permutation",
+                path);
+  if (!fileProw.empty()) {
+    if (!is_perm(fileProw))
+      return fail(PVGLU_E_ARG,
+                  "plan_load: stored Prow of %s is not a permutation", path);
+    for (int i = 0; i < n; i++)
+      if (fileProw[(size_t)fileProw[(size_t)i]] != i)
+        return fail(PVGLU_E_ARG,
+                    "plan_load: stored Prow of %s is not an involution", path);
+  }
   pvglu_plan *plan = nullptr;
   int st = pvglu_plan_create(n, rp.data(), ci.data(), &plan);
   if (st)

@@ -40,6 +40,7 @@ struct pvglu_pv {
   pvglu::SparseMap lin_A;
   std::vector<double> lin_b;
 
+  int form = PVGLU_NEWTON_CORRECTION; // Newton step form of the drivers
   pvglu::NewtonDev *nd = nullptr;
   pvglu::DropinDev *dd = nullptr;
   pvglu::CloudDev *cd = nullptr;

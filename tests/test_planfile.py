@@ -92,3 +92,40 @@ def test_arrays_that_disagree_with_the_pattern_are_rejected(tmp_path):
     bad.write_bytes(bytes(raw))
     with pytest.raises(pv.PvgluError, match="differs"):
         pv.Plan.load(bad)
+
+
+def _rewrite_section(raw_file, sid_target, edit):
+    """Applies edit(words_of_section) to one section and redoes the checksum."""
+    raw = bytearray(raw_file[:-8])
+    words = np.frombuffer(bytes(raw[24:]), dtype=np.int32).copy()
+    pos = 0
+    while True:
+        sid, ln = words[pos], words[pos + 1]
+        if sid == sid_target:
+            edit(words[pos + 2:pos + 2 + ln])
+            break
+        pos += 2 + ln
+    raw[24:] = words.tobytes()
+    raw += _fnv(bytes(raw)).to_bytes(8, "little")
+    return bytes(raw)
+
+
+@pytest.mark.parametrize("sid,edit,match", [
+    (100, lambda w: w.__setitem__(0, w[1]), "P of .* is not a permutation"),     # duplicate
+    (100, lambda w: w.__setitem__(3, len(w)), "P of .* is not a permutation"),   # out of range
+    (101, lambda w: w.__setitem__(slice(0, 3), [1, 2, 0]), "not an involution"),  # 3-cycle
+    (101, lambda w: w.__setitem__(0, -1), "Prow of .* is not a permutation"),
+])
+def test_crafted_orderings_are_rejected(tmp_path, sid, edit, match):
+    """The checksum catches accidents only: a file whose ordering section was
+    altered and re-checksummed fails on the ordering's own checks (P a
+    permutation of [0, n); Prow a permutation that is its own inverse)."""
+    plan, P, Prow = _plant_plan()
+    f = tmp_path / "p.pvglu"
+    plan.save(f, P, Prow)
+    bad = tmp_path / "crafted.pvglu"
+    bad.write_bytes(_rewrite_section(f.read_bytes(), sid, edit))
+    with pytest.raises(pv.PvgluError, match=match):
+        pv.Plan.load(bad)
+    # the unaltered file still loads
+    pv.Plan.load(f)
