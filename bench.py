@@ -165,74 +165,178 @@ def _time_ms(torch, fn, reps):
     return e0.elapsed_time(e1) / reps
 
 
+def _oracle_newton(pv, plant):
+    """The oracle Newton of a plant (correction form, FP64 CPU)."""
+    from oracle import oracle as O
+    pos, neg, rsh = plant.modules()
+    a, b, g = plant.cables()
+    rp, ci = plant.pattern()
+    P, Prow = pv.reorder(plant.n, rp, ci)
+    return O.PVNewton(plant.n, pos, neg, rsh, a, b, g, plant.buses(), P, Prow,
+                      form="correction")
+
+
+def newton_error(on, plant, x, v, G, ks, cold=False):
+    """max |x - x_oracle| (V) over instances ks: the oracle Newton iterated on
+    its own to 1e-12 V, from the library's cold start (cold=True) or from the
+    device's converged element voltages (it then runs until its own last
+    update is below 1e-12 V, i.e. to its own fixed point)."""
+    worst = 0.0
+    for k in ks:
+        v0 = plant.v0() if cold else v[:, k]
+        xo, _, _ = on.solve(9e-3 * G[:, k], v0, ladder=False, tol=1e-12)
+        worst = max(worst, float(np.abs(x[:, k] - xo).max()))
+    return worst
+
+
+def _spread(its):
+    its = np.asarray(its)
+    return [int(its.min()), int(np.median(its)), int(its.max())]
+
+
 def config_legs(torch, pv, dev):
     """Secondary workloads, reported beside the headline value:
-    mixed: 128 instances round-robin over 10 / 40 / 100-inverter plants, one
-    multi-pattern launch for refactor+solve and one for the adjoint solve;
-    large: the 400-inverter plant (269,200 unknowns), 32 scenarios, five
-    warm-started one-second time steps of batched Newton, one refactor+solve
-    launch, and the energy adjoint of one step."""
-    out = {}
+    cfg0: the 4 x 10 x 20 plant, drop-in Newton (the reference's call
+    sequence) against the oracle;
+    cfg1 window: the 40 x 24 x 28 plant, one scenario, 200 one-second time
+    steps, each a warm-started Newton solve on the single-instance device path;
+    mixed (cfg3): 128 instances round-robin over 10 / 40 / 100-inverter
+    plants, batched Newton per plant, one multi-pattern launch for
+    refactor+solve and one for the adjoint solve;
+    large (cfg4): the 400-inverter plant (269,200 unknowns), 32 scenarios,
+    50 one-second steps of batched Newton with the energy adjoint of each."""
+    out, acc = {}, {}
+    # ------------------------------------------------------------ cfg0
+    p0 = pv.PV(4, 10, 20, seed=0)
+    g0 = p0.irradiance(-1)
+    x0, v0, st0 = p0.newton_dropin(g0, tol=1e-12, ladder=True)
+    on0 = _oracle_newton(pv, p0)
+    acc["cfg0_4x10x20"] = {
+        "max_abs_dx_vs_oracle_V": newton_error(on0, p0, x0[:, None], v0[:, None],
+                                               g0[:, None], [0], cold=True),
+        "instances": 1, "checked": 1, "oracle_start": "cold (ladder off, 1e-12 V)",
+        "path": "drop-in (reference call sequence), correction form, ladder",
+        "iterations": int(st0[0])}
+    del p0, on0
+    # ------------------------------------------------------------ cfg1
+    p1 = pv.PV(40, 24, 28, seed=0)
+    steps = 200
+    v = None
+    its, xs = [], {}
+    t0 = time.perf_counter()
+    loop_ms = 0.0
+    for t in range(steps):
+        G = p1.irradiance(0, float(t))[:, None].copy()
+        x, v, p, st = p1.newton_batched(G, v=v, tol=1e-12, ladder=False)
+        its.append(int(st[0]))
+        loop_ms += float(st[1])
+        if t in (0, steps - 1):
+            xs[t] = (x, v, G)
+    wall = time.perf_counter() - t0
+    on1 = _oracle_newton(pv, p1)
+    e_first = newton_error(on1, p1, *xs[0], [0], cold=True)
+    e_last = newton_error(on1, p1, *xs[steps - 1], [0])
+    out["window_cfg1_40x24x28"] = {
+        "steps": steps, "wall_s": round(wall, 2), "ms_per_step": round(wall / steps * 1e3, 2),
+        "newton_loop_ms_per_step": round(loop_ms / steps, 2),
+        "iterations_per_step": _spread(its), "iterations_total": int(sum(its)),
+        "note": "one cloud scenario, 200 one-second steps, each warm-started from the "
+                "previous step's state; single-instance device path (batch 1: device "
+                "stamping, COMPONENT refactor+solve, graph replays); host cloud field per step"}
+    acc["cfg1_40x24x28_window"] = {
+        "max_abs_dx_vs_oracle_V": max(e_first, e_last), "instances": steps, "checked": 2,
+        "oracle_start": "cold at t = 0; from the device state at t = 199",
+        "path": "single-instance device Newton, correction form"}
+    del p1, on1
+    # ------------------------------------------------------------ cfg3
     plants = [pv.PV(I, 24, 28, seed=3) for I in (10, 40, 100)]
     batches = [len(range(g, 128, 3)) for g in range(3)]
-    As, bs, gs, lus, xs, ys, nbytes, abytes = [], [], [], [], [], [], 0, 0
+    As, bs, gs, lus, xs_, ys, nbytes, abytes = [], [], [], [], [], [], 0, 0
+    worst, floor, nits = 0.0, 0, []
     for g, (p, B) in enumerate(zip(plants, batches)):
         plan = p.plan()
         G = np.stack([p.irradiance(s, 0.0) for s in range(g, 128, 3)], axis=1)
+        x, v, _, _ = p.newton_batched(G, tol=1e-12, ladder=False)
+        it, last = p.newton_info(B)
+        floor += int((last > 1e-12).sum())
+        nits.extend(it.tolist())
+        worst = max(worst, newton_error(_oracle_newton(pv, p), p, x, v, G, [B // 2]))
         A = torch.empty((plan.nnz, B), dtype=torch.float64, device=dev)
         b = torch.empty((p.n, B), dtype=torch.float64, device=dev)
-        p.stamp(B, torch.from_numpy(G).to(dev), torch.from_numpy(p.v0(B)).to(dev), A, b, -1.0)
+        p.stamp(B, torch.from_numpy(G).to(dev), torch.from_numpy(v).to(dev), A, b, -1.0)
         As.append(A), bs.append(b), gs.append(b.clone())
-        lus.append(torch.empty_like(A)), xs.append(torch.empty_like(b)), ys.append(torch.empty_like(b))
+        lus.append(torch.empty_like(A)), xs_.append(torch.empty_like(b))
+        ys.append(torch.empty_like(b))
         nbytes += B * (24 * plan.nnz + 16 * p.n) + 4 * plan.nnz + 8 * p.n
         abytes += B * (8 * plan.nnz + 16 * p.n)
     m = pv.Multi([p.plan() for p in plants], batches)
-    t_rs = _time_ms(torch, lambda: m.refactor_solve(As, lus, bs, xs), 20)
+    t_rs = _time_ms(torch, lambda: m.refactor_solve(As, lus, bs, xs_), 20)
     t_adj = _time_ms(torch, lambda: m.solve_transpose(lus, gs, ys), 20)
     out["mixed_10_40_100"] = {
         "instances": 128, "refactor_solve_ms": round(t_rs, 4),
         "refactor_solve_per_s": round(128 / t_rs * 1e3, 1),
         "refactor_solve_GBps": round(nbytes / t_rs / 1e6, 1),
-        "adjoint_ms": round(t_adj, 4), "adjoint_GBps": round(abytes / t_adj / 1e6, 1)}
-    del As, bs, gs, lus, xs, ys, m
+        "adjoint_ms": round(t_adj, 4), "adjoint_GBps": round(abytes / t_adj / 1e6, 1),
+        "note": "systems stamped at each instance's converged Newton state"}
+    acc["cfg3_mixed_10_40_100"] = {
+        "max_abs_dx_vs_oracle_V": worst, "instances": 128, "checked": 3,
+        "noise_floor_stops": floor, "iterations_per_instance": _spread(nits),
+        "oracle_start": "from the device state (one instance per plant size)",
+        "path": "batched device Newton per plant, correction form"}
+    del As, bs, gs, lus, xs_, ys, m, plants
+    # ------------------------------------------------------------ cfg4
     big = pv.PV(400, 24, 28, seed=4)
-    B = 32
-    t0 = time.perf_counter()
+    B, steps = 32, 50
+    gdev = torch.empty((big.nmod, B), dtype=torch.float64, device=dev)
     v = None
-    iters, loop_ms = [], []
-    for t in range(6):
-        G = np.stack([big.irradiance(s, float(t)) for s in range(B)], axis=1)
-        x, v, p, st = big.newton_batched(G, v=v, tol=1e-10, ladder=False)
-        iters.append(int(st[0]))
-        loop_ms.append(float(st[1]))
+    energy = np.zeros(B)
+    gnorm2 = 0.0
+    its, floor, step_ms = [], 0, []
+    first = None
+    t0 = time.perf_counter()
+    for t in range(steps):
+        ts = time.perf_counter()
+        big.irradiance_device(list(range(B)), [float(t)] * B, gdev)
+        G = gdev.cpu().numpy()
+        # forward Newton (warm from the previous second) + energy adjoint:
+        # E = sum_t P_t * 1 s, so dE/dG_{m,t} = dP_t/dG_{m,t}
+        p, dG, dgc, v = big.gradients(G, v=v, tol=1e-12)
+        it, last = big.newton_info(B)
+        its.append(_spread(it))
+        floor += int((last > 1e-12).sum())
+        energy += p * 1.0
+        gnorm2 += float(np.sum(dG * dG))
+        step_ms.append((time.perf_counter() - ts) * 1e3)
         if t == 0:
-            t0 = time.perf_counter()          # steps 1..5 are the warm-started ones
-    t_step = (time.perf_counter() - t0) / 5 * 1e3
-    t_loop = sum(loop_ms[1:]) / 5
-    ms_iter = sum(loop_ms[1:]) / max(1, sum(iters[1:]))
+            first = (G.copy(), v.copy())
+    wall = time.perf_counter() - t0
     plan = big.plan()
+    Gl, vl = first
     A = torch.empty((plan.nnz, B), dtype=torch.float64, device=dev)
     b = torch.empty((big.n, B), dtype=torch.float64, device=dev)
-    big.stamp(B, torch.from_numpy(G).to(dev), torch.from_numpy(v).to(dev), A, b, -1.0)
+    big.stamp(B, torch.from_numpy(Gl).to(dev), torch.from_numpy(vl).to(dev), A, b, -1.0)
     lu, xx = torch.empty_like(A), torch.empty_like(b)
     t_rs = _time_ms(torch, lambda: plan.refactor_solve(A, lu, b, xx, B), 20)
     byts = B * (24 * plan.nnz + 16 * big.n) + 4 * plan.nnz + 8 * big.n
-    t1 = time.perf_counter()
-    big.gradients(G, v=v, tol=1e-12)
-    t_adj = (time.perf_counter() - t1) * 1e3
+    # accuracy at t = 0 (two scenarios)
+    x0, v0, _, _ = big.newton_batched(Gl, v=vl, tol=1e-12, ladder=False)
+    e4 = newton_error(_oracle_newton(pv, big), big, x0, v0, Gl, [3, 30])
     out["large_400x24x28"] = {
-        "n": big.n, "scenarios": B, "warm_step_ms": round(t_step, 2),
-        "newton_loop_ms_per_step": round(t_loop, 2),
-        "newton_ms_per_iteration": round(ms_iter, 4),
-        "newton_iterations_per_step": iters,
+        "n": big.n, "scenarios": B, "steps": steps, "window_wall_s": round(wall, 2),
+        "ms_per_step": round(float(np.median(step_ms)), 1),
+        "iterations_per_step_min_median_max": its[:3] + ["..."] + its[-2:],
+        "energy_MWh_per_scenario_mean": round(float(energy.mean()) / 3.6e9, 6),
+        "dE_dG_norm_Ws_per_Wm2": round(gnorm2 ** 0.5, 3),
         "refactor_solve_ms": round(t_rs, 4), "refactor_solve_GBps": round(byts / t_rs / 1e6, 1),
-        "energy_adjoint_step_ms": round(t_adj, 2),
-        "note": "warm_step_ms: one warm-started second end to end (host cloud field for "
-                "32 x 268,800 modules, copies, batched device Newton to 1e-10 V from the "
-                "previous second's state); newton_loop_ms_per_step: the device Newton loop "
-                "alone (graph replays and polls); energy_adjoint_step_ms: re-converge to "
-                "1e-12 V + transposed solves + gradient kernels"}
-    return out
+        "note": "per step: device cloud field (32 x 268,800 modules), batched Newton warm-"
+                "started from the previous second to 1e-12 V, energy adjoint dP_t/dG_{m,t} "
+                "for every module (transposed solves with the step's factors); "
+                "ms_per_step includes the host copies of G, v and dE/dG"}
+    acc["cfg4_400x24x28"] = {
+        "max_abs_dx_vs_oracle_V": e4, "instances": B * steps, "checked": 2,
+        "noise_floor_stops": floor, "oracle_start": "from the device state (t = 0)",
+        "path": "batched device Newton, correction form"}
+    return out, acc
 
 
 def main():
@@ -258,10 +362,16 @@ def main():
     # Converged operating point of each scenario (batched device Newton)
     G = np.stack([plant.irradiance(s, 0.0) for s in range(B)], axis=1)
     t0 = time.perf_counter()
-    state = "converged Newton state (max|dV| <= 1e-10 V)"
+    state = "converged Newton state (max|dV| <= 1e-12 V, or <= 1e-11 V at the noise floor)"
+    newton = None
     try:
-        x_n, v_n, power, nstats = plant.newton_batched(G, tol=1e-10, ladder=False)
+        x_n, v_n, power, nstats = plant.newton_batched(G, tol=1e-12, ladder=False)
         t_newton = time.perf_counter() - t0
+        its, last = plant.newton_info(B)
+        newton = {"replays": int(nstats[0]), "ms_per_iteration": round(float(nstats[2]), 4),
+                  "iterations_per_instance": _spread(its),
+                  "noise_floor_stops": int((last > 1e-12).sum()),
+                  "max_final_update_V": float(last.max())}
         print(f"[bench] batched Newton: {nstats[0]:.0f} iterations, {nstats[2]:.3f} ms/iter, "
               f"max|dV| {nstats[3]:.2e} V, total power {power.sum() / 1e6:.2f} MW "
               f"({t_newton:.1f} s incl. setup)", flush=True)
@@ -370,6 +480,17 @@ def main():
                       "schedule": sched_name,
                       "newton_tol_V": 1e-10, "max_abs_dx_vs_oracle": max_dx},
            "roofline": roofline}
+    if newton is not None:
+        # per-config Newton accuracy: node voltages against the FP64 oracle Newton
+        on = _oracle_newton(pv, plant)
+        ks = [0, B // 3, 2 * B // 3, B - 1]
+        out["newton_accuracy"] = {"cfg2_40x24x28_b512": dict(
+            newton, max_abs_dx_vs_oracle_V=newton_error(on, plant, x_n, v_n, G, ks),
+            instances=B, checked=len(ks), oracle_start="from the device state",
+            path="batched device Newton (hipGraph per iteration), correction form")}
+        del on
+    else:
+        out["newton_accuracy"] = {}
 
     # ------------------------------------------------------------- plain
     if not args.no_plain:
@@ -441,9 +562,12 @@ def main():
         out["batched_over_cpu_no_fast_math"] = value / nb["value"]
 
     if not args.no_configs:
-        out["configs"] = config_legs(torch, pv, dev)
+        out["configs"], acc = config_legs(torch, pv, dev)
+        out["newton_accuracy"].update(acc)
         for k, v in out["configs"].items():
             print(f"[bench] {k}: {v}", flush=True)
+        for k, v in acc.items():
+            print(f"[bench] accuracy {k}: {v}", flush=True)
 
     if args.copy_probe:
         src = torch.empty(1 << 27, dtype=torch.float64, device=dev)

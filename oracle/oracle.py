@@ -349,6 +349,9 @@ class PVNewton:
         self.abase = np.zeros(len(fci))
         for (i, j), v in base.items():
             self.abase[slot(i, j)] += v
+        # slots of each cable's four stamps (a,a), (b,b), (a,b), (b,a)
+        self.cab_slots = np.array([[slot(a, a), slot(b, b), slot(a, b), slot(b, a)]
+                                   for a, b in zip(self.cab_a, self.cab_b)], dtype=np.int64)
         sl, sg, el = [], [], []
         for e in range(len(pos)):
             p, m = int(pos[e]), int(neg[e])
@@ -490,6 +493,38 @@ class PVNewton:
         self.lu = lu
         self.last_max = mx
         return x, v, it
+
+    def with_cable(self, c, g):
+        """A copy whose cable c has conductance g (the other data shared)."""
+        import copy
+        other = copy.copy(self)
+        other.abase = self.abase.copy()
+        other.cab_g = self.cab_g.copy()
+        other.abase[self.cab_slots[c]] += (g - self.cab_g[c]) * np.array([1.0, 1.0, -1.0, -1.0])
+        other.cab_g[c] = g
+        return other
+
+    def adjoint(self, x, v, iph):
+        """Gradients of P = sum_bus V G_N (V - 600) at a converged state (x, v):
+        A(v)^T lam = dP/dx with the oracle's own factors (U^T then L^T), then
+        dP/dG_m = 9e-3 dI/dIph (lam[+] - lam[-]) and
+        dP/dg_c = -(lam[a] - lam[b]) (x_a - x_b).  Returns (dP/dG, dP/dg)."""
+        _, dIdV, dIdIph = self.module(v, iph)
+        vals = self._values(-dIdV, -1.0)
+        st, bad = glu_csc_sequential(self.s, vals)
+        if st != 0:
+            raise RuntimeError(f"oracle GLU status {st} at column {bad}")
+        gx = np.zeros(self.n)
+        gx[self.buses] = self.GN * (2.0 * x[self.buses] - self.VN)
+        g = np.zeros(self.n)
+        g[self.colmap] = gx
+        solve_transpose_csc(self.s, vals, g)
+        lam = g[self.rowmap]
+        lp = np.where(self.pos >= 0, lam[np.maximum(self.pos, 0)], 0.0)
+        ln = np.where(self.neg >= 0, lam[np.maximum(self.neg, 0)], 0.0)
+        dG = 9e-3 * dIdIph * (lp - ln)
+        dg = -(lam[self.cab_a] - lam[self.cab_b]) * (x[self.cab_a] - x[self.cab_b])
+        return dG, dg
 
     def power(self, x, buses=None):
         V = x[self.buses if buses is None else np.asarray(buses)]

@@ -69,7 +69,7 @@ extern "C" int pvglu_pv_irradiance_device(pvglu_pv *pv, int batch,
                                           const int64_t *scenarios,
                                           const double *times, double *g_dev,
                                           void *stream) {
-  clear_error();
+  begin_call();
   if (!pv || batch <= 0 || !scenarios || !times || !g_dev)
     return fail(PVGLU_E_ARG, "pv_irradiance_device: bad arguments");
   int ndev = 0;

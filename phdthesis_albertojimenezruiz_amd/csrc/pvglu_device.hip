@@ -153,6 +153,11 @@ static int upload_levels(DevicePlan *d, const KernelLevels &k, DevLevels &L) {
   return PVGLU_OK;
 }
 
+void begin_call() {
+  clear_error();
+  (void)hipGetLastError();
+}
+
 void device_plan_destroy(pvglu_plan *plan) {
   if (!plan->dev)
     return;
@@ -267,7 +272,7 @@ static int setup_groups(pvglu_plan *P) {
 }
 
 extern "C" int pvglu_plan_upload(pvglu_plan *P) {
-  clear_error();
+  begin_call();
   if (!P)
     return fail(PVGLU_E_ARG, "plan_upload: null plan");
   if (P->dev)
@@ -586,7 +591,7 @@ static int launch_component(pvglu_plan *P, int W, int mode, int threads,
 // mode: 0 refactor, 1 refactor+solve, 2 solve, 3 transposed solve
 static int run_numeric(pvglu_plan *P, int mode, const double *A, double *LU,
                        const double *b, double *x, int B, void *stream) {
-  clear_error();
+  begin_call();
   int st = require_device(P);
   if (st)
     return st;
@@ -628,7 +633,7 @@ extern "C" int pvglu_solve_transpose(pvglu_plan *P, const double *lu,
 
 extern "C" int pvglu_status(pvglu_plan *P, int *code, int *column,
                             int *instance) {
-  clear_error();
+  begin_call();
   int st = require_device(P);
   if (st)
     return st;
@@ -645,7 +650,7 @@ extern "C" int pvglu_status(pvglu_plan *P, int *code, int *column,
 }
 
 extern "C" int pvglu_status_reset(pvglu_plan *P, void *stream) {
-  clear_error();
+  begin_call();
   int st = require_device(P);
   if (st)
     return st;
@@ -656,7 +661,7 @@ extern "C" int pvglu_status_reset(pvglu_plan *P, void *stream) {
 }
 
 extern "C" int pvglu_set_schedule(pvglu_plan *P, int variant, int width) {
-  clear_error();
+  begin_call();
   if (!P)
     return fail(PVGLU_E_ARG, "set_schedule: null plan");
   if (variant < PVGLU_SCHED_AUTO || variant > PVGLU_SCHED_HYBRID)
@@ -670,7 +675,7 @@ extern "C" int pvglu_set_schedule(pvglu_plan *P, int variant, int width) {
 }
 
 extern "C" int pvglu_set_stream_groups(pvglu_plan *P, int groups) {
-  clear_error();
+  begin_call();
   if (!P)
     return fail(PVGLU_E_ARG, "set_stream_groups: null plan");
   if (groups != 1 && groups != 2 && groups != 4)
@@ -680,7 +685,7 @@ extern "C" int pvglu_set_stream_groups(pvglu_plan *P, int groups) {
 }
 
 extern "C" int pvglu_set_cache_policy(pvglu_plan *P, int policy) {
-  clear_error();
+  begin_call();
   if (!P)
     return fail(PVGLU_E_ARG, "set_cache_policy: null plan");
   if (policy != PVGLU_CACHE_AUTO && policy != PVGLU_CACHE_STREAMING &&
@@ -716,7 +721,7 @@ extern "C" int pvglu_newton_update(pvglu_plan *P, int ne, int batch,
                                    const double *x, double *v_elem,
                                    double *delta_sum, double *delta_max,
                                    const int *active, void *stream) {
-  clear_error();
+  begin_call();
   int st = require_device(P);
   if (st)
     return st;
@@ -770,7 +775,7 @@ static std::map<std::pair<uint64_t, int>, TuneChoice> g_tune_cache;
 
 extern "C" int pvglu_autotune(pvglu_plan *P, int batch, int reps,
                               double *best_ms) {
-  clear_error();
+  begin_call();
   int st = require_device(P);
   if (st)
     return st;
@@ -911,7 +916,7 @@ struct pvglu_multi {
 
 extern "C" int pvglu_multi_create(int ngroups, pvglu_plan *const *plans,
                                   const int *batches, pvglu_multi **out) {
-  clear_error();
+  begin_call();
   if (!out || ngroups <= 0 || ngroups > kMaxGroups || !plans || !batches)
     return fail(PVGLU_E_ARG, "multi_create: 1..%d groups required", kMaxGroups);
   auto *m = new pvglu_multi();
@@ -1069,7 +1074,7 @@ static int launch_multi_steps_w(pvglu_multi *m, int mode,
 static int launch_multi(pvglu_multi *m, int mode, const double *const *a,
                         double *const *lu, const double *const *b,
                         double *const *x, void *stream) {
-  clear_error();
+  begin_call();
   if (!m || !lu || !b || !x || (mode != 3 && !a))
     return fail(PVGLU_E_ARG, "multi: null pointer arrays");
   for (int g = 0; g < m->ngroups; g++)

@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(1024) k_scan_inclusive(int *a, int n) {
 }
 
 extern "C" int pvglu_inclusive_scan_device(int *d, int n, void *stream) {
-  clear_error();
+  begin_call();
   if (!d || n < 0)
     return fail(PVGLU_E_ARG, "inclusive_scan: bad arguments");
   if (n == 0)
@@ -253,7 +253,7 @@ extern "C" int pvglu_inclusive_scan_device(int *d, int n, void *stream) {
 extern "C" int pvglu_obtain_fillin(int n, const int *rowptr, const int *colidx,
                                    int **filled_rowptr, int **filled_colidx,
                                    int *filled_nnz, void *stream) {
-  clear_error();
+  begin_call();
   if (n <= 0 || !rowptr || !colidx || !filled_rowptr || !filled_colidx)
     return fail(PVGLU_E_ARG, "obtain_fillin: bad arguments");
   for (int i = 0; i < n; i++) {

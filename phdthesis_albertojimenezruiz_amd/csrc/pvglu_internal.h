@@ -18,6 +18,12 @@ constexpr int kStarMaxUpdates = 32;
 // Thread-local error message + status helper.
 int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 void clear_error();
+// clear_error() plus discarding a HIP error recorded by an earlier runtime
+// call outside this entry point (another library's, or a teardown's that
+// ignores its status): the entry points that launch kernels check
+// hipGetLastError() after their launches, and that check must see only
+// their own.  Defined in pvglu_device.hip.
+void begin_call();
 
 // Levels of chains of columns in the flattened form of
 // PV/008_convert_chains_columns_levels_into_vector.cpp:4-30.

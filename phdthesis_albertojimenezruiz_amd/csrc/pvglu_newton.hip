@@ -1010,7 +1010,7 @@ extern "C" int pvglu_pv_newton_batched(pvglu_pv *pv, int batch, const double *g,
                                        double *power_out, double tol,
                                        int max_iters, int ladder,
                                        double *stats) {
-  clear_error();
+  begin_call();
   if (!pv || batch <= 0 || !g || !v_module)
     return fail(PVGLU_E_ARG, "newton_batched: bad arguments");
   int st = batched_solve(pv, batch, g, v_module, tol,
@@ -1024,7 +1024,7 @@ extern "C" int pvglu_pv_gradients(pvglu_pv *pv, int batch, const double *g,
                                   double *v_module, double *power_out,
                                   double *dP_dG, double *dP_dgc, double tol,
                                   int max_iters) {
-  clear_error();
+  begin_call();
   if (!pv || batch <= 0 || !g || !v_module)
     return fail(PVGLU_E_ARG, "gradients: bad arguments");
   int st = batched_solve(pv, batch, g, v_module, tol,
@@ -1075,7 +1075,7 @@ extern "C" int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g,
                                       double *v_module, double *x_out,
                                       double tol, int ladder, int fixed_iters,
                                       double *stats) {
-  clear_error();
+  begin_call();
   if (!pv || !g || !v_module)
     return fail(PVGLU_E_ARG, "newton_dropin: bad arguments");
   int st = pv_prepare(pv);
@@ -1212,7 +1212,7 @@ extern "C" int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g,
 // ------------------------------------------------------ stamping entry points
 
 extern "C" pvglu_plan *pvglu_pv_plan(pvglu_pv *pv) {
-  clear_error();
+  begin_call();
   if (!pv) {
     fail(PVGLU_E_ARG, "pv_plan: null");
     return nullptr;
@@ -1225,7 +1225,7 @@ extern "C" pvglu_plan *pvglu_pv_plan(pvglu_pv *pv) {
 extern "C" int pvglu_pv_stamp(pvglu_pv *pv, int batch, const double *g,
                               const double *v, double g_virtual, double *a,
                               double *b, void *stream) {
-  clear_error();
+  begin_call();
   if (!pv || batch <= 0 || !g || !v || !a || !b)
     return fail(PVGLU_E_ARG, "pv_stamp: bad arguments");
   int st = newton_setup(pv);
@@ -1260,7 +1260,7 @@ extern "C" int pvglu_pv_stamp(pvglu_pv *pv, int batch, const double *g,
 }
 
 extern "C" int pvglu_pv_set_newton_form(pvglu_pv *pv, int form) {
-  clear_error();
+  begin_call();
   if (!pv || (form != PVGLU_NEWTON_CORRECTION && form != PVGLU_NEWTON_DIRECT))
     return fail(PVGLU_E_ARG, "set_newton_form: bad arguments");
   pv->form = form;
@@ -1275,7 +1275,7 @@ extern "C" int pvglu_pv_get_newton_form(const pvglu_pv *pv) {
 
 extern "C" int pvglu_pv_newton_info(const pvglu_pv *pv, int batch, int *iters,
                                     double *last_max_delta) {
-  clear_error();
+  begin_call();
   if (!pv || !pv->nd || batch <= 0 || batch > pv->nd->B)
     return fail(PVGLU_E_STATE, "newton_info: no batched solve of this size");
   if (iters)

@@ -194,3 +194,56 @@ def test_correction_form_first_iterate_is_the_direct_solve(plant):
     xd, vd, *_ = on_d.iterate(v, iph, 100.0)
     xc, vc, *_ = on_c.iterate(v, iph, 100.0, np.zeros(plant.n))
     assert np.array_equal(xd, xc) and np.array_equal(vd, vc)
+
+
+def _inverter_power(plant, x, iph, inv):
+    """P of inverter inv from its strings' bottom-module currents (equal to
+    V_bus G_N (V_bus - 600) by KCL, without the cancellation in V_bus - 600)."""
+    pos, _, rsh = plant.modules()
+    S, M = plant.S, plant.M
+    tot = sum(O.module_current(x[pos[inv * S * M + s * M]], iph[inv * S * M + s * M],
+                               I0, A_N, RS, rsh[inv * S * M + s * M])[0] for s in range(S))
+    return x[plant.buses()[inv]] * tot
+
+
+def test_oracle_adjoint_against_central_differences(plant):
+    """PVNewton.adjoint (its own factors, U^T then L^T) against Richardson
+    central differences of the owning inverter's power, with the oracle
+    Newton re-converged to 1e-13 V at each perturbed parameter: modules
+    (h = 0.1, 0.05 W/m^2) and cables (PVNewton.with_cable, h = 0.02, 0.01 S)."""
+    on = _oracle(plant, "correction")
+    iph = 9e-3 * plant.irradiance(1, 0.0)
+    x, v, _ = on.solve(iph, plant.v0(), ladder=False, tol=1e-13)
+    dG, dg = on.adjoint(x, v, iph)
+    S, M = plant.S, plant.M
+    rng = np.random.default_rng(8)
+    for m in rng.choice(plant.nmod, 4, replace=False):
+        inv = m // (S * M)
+
+        def fd(h):
+            out = []
+            for sg in (1, -1):
+                i2 = iph.copy()
+                i2[m] += sg * 9e-3 * h
+                xx, _, _ = on.solve(i2, v, ladder=False, tol=1e-13)
+                out.append(_inverter_power(plant, xx, i2, inv))
+            return (out[0] - out[1]) / (2 * h)
+
+        want = (4 * fd(0.05) - fd(0.1)) / 3
+        assert abs(dG[m] - want) <= 1e-6 * abs(want) + 1e-8, (m, dG[m], want)
+    a, b, gc = plant.cables()
+    for c in rng.choice(plant.ncab, 3, replace=False):
+        inv = c // S
+
+        def fdc(h):
+            out = []
+            for sg in (1, -1):
+                o2 = on.with_cable(c, gc[c] + sg * h)
+                xx, _, _ = o2.solve(iph, v, ladder=False, tol=1e-13)
+                out.append(_inverter_power(plant, xx, iph, inv))
+            return (out[0] - out[1]) / (2 * h)
+
+        want = (4 * fdc(0.01) - fdc(0.02)) / 3
+        assert abs(dg[c] - want) <= 1e-6 * abs(want) + 3e-8, (c, dg[c], want)
+    # with_cable leaves the original untouched
+    assert np.array_equal(on.cab_g, gc)
