@@ -61,6 +61,24 @@ int pvglu_obtain_fillin(int n, const int *rowptr, const int *colidx,
                         int **filled_rowptr, int **filled_colidx,
                         int *filled_nnz, void *stream);
 
+/* Fill-in discovery that allocates nothing.  No counterpart in the
+ * reference: for callers that cannot call hipMalloc (code under hipGraph
+ * capture elsewhere in their stream, tight serving loops, harnesses that
+ * forbid device allocation).  Same passes and result as
+ * pvglu_obtain_fillin, with every device buffer in one file-scope device
+ * array of the library (about 21 MB, held on each device the library runs
+ * on until the process ends) and the output written to caller buffers:
+ * filled_rowptr (n + 1 ints) and filled_colidx (capacity ints).  Limits:
+ * n <= 524288 rows and 2097152 nonzeros in the input, every intermediate
+ * and the filled pattern; beyond them the call fails with PVGLU_E_ARG and a
+ * message naming the limit.  When the filled pattern has more than capacity
+ * nonzeros the call fails with PVGLU_E_ARG and *filled_nnz holds the size
+ * needed.  Synchronizes the stream once per pass (as pvglu_obtain_fillin).
+ * Calls that overlap on one device share the array and race. */
+int pvglu_obtain_fillin_noalloc(int n, const int *rowptr, const int *colidx,
+                                int *filled_rowptr, int *filled_colidx,
+                                int capacity, int *filled_nnz, void *stream);
+
 /* Device prefix sum (inclusive, in place) over n int32 values resident on
  * the device.  Replaces call_cumsum_CUDA (PV/005b_cumsum_CUDA.cu:22-31).
  * Stream-ordered; one workgroup walks the array in 4096-value tiles with a

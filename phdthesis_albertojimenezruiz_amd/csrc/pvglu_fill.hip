@@ -19,6 +19,7 @@
 
 #include "pvglu_internal.h"
 
+#include <mutex>
 #include <vector>
 
 using namespace pvglu;
@@ -355,5 +356,136 @@ extern "C" int pvglu_obtain_fillin(int n, const int *rowptr, const int *colidx,
   *filled_colidx = h_ci;
   if (filled_nnz)
     *filled_nnz = nnz;
+  return PVGLU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// pvglu_obtain_fillin_noalloc: the same passes with every device buffer in
+// one file-scope array, for callers that cannot allocate device memory.
+// Capacity: n <= kNaMaxN rows and kNaMaxNnz nonzeros of any intermediate or
+// the final filled pattern; the array (about 21 MB) is held on every device
+// the library runs on until the process ends.  Layout (int words): two row
+// pointer arrays, the per-row counts, the active-row mask (bytes) and two
+// column index arrays, swapped between passes.  Calls that overlap on one
+// device share the array and race.
+static constexpr int kNaMaxN = 1 << 19;
+static constexpr int kNaMaxNnz = 1 << 21;
+static constexpr size_t kNaRp = (size_t)kNaMaxN + 64; // n + 1, padded
+static constexpr size_t kNaWords =
+    2 * kNaRp + kNaMaxN + kNaMaxN / 4 + 2 * (size_t)kNaMaxNnz;
+static __device__ __align__(256) int scratch[kNaWords];
+
+// Device address of `scratch` per device, resolved once per device (with
+// that device current) on the first call there.
+static std::mutex g_na_mu;
+static int *g_na_ptr[64];
+
+static int na_scratch(int **out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64)
+    return fail(PVGLU_E_HIP, "obtain_fillin_noalloc: device %d unsupported", dev);
+  std::lock_guard<std::mutex> g(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void *p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_na_ptr[dev] = (int *)p;
+  }
+  *out = g_na_ptr[dev];
+  return PVGLU_OK;
+}
+
+extern "C" int pvglu_obtain_fillin_noalloc(int n, const int *rowptr,
+                                           const int *colidx,
+                                           int *filled_rowptr,
+                                           int *filled_colidx, int capacity,
+                                           int *filled_nnz, void *stream) {
+  begin_call();
+  if (n <= 0 || !rowptr || !colidx || !filled_rowptr || !filled_colidx ||
+      !filled_nnz || capacity < 0)
+    return fail(PVGLU_E_ARG, "obtain_fillin_noalloc: bad arguments");
+  if (n > kNaMaxN)
+    return fail(PVGLU_E_ARG,
+                "obtain_fillin_noalloc: n = %d exceeds the maximum of %d rows",
+                n, kNaMaxN);
+  if (rowptr[0] != 0 || rowptr[n] < 0 || rowptr[n] > kNaMaxNnz)
+    return fail(PVGLU_E_ARG,
+                "obtain_fillin_noalloc: %d nonzeros exceed the maximum of %d",
+                rowptr[n], kNaMaxNnz);
+  for (int i = 0; i < n; i++) {
+    if (rowptr[i + 1] < rowptr[i])
+      return fail(PVGLU_E_ARG, "obtain_fillin_noalloc: rowptr not monotone");
+    for (int p = rowptr[i]; p < rowptr[i + 1]; p++)
+      if (colidx[p] < 0 || colidx[p] >= n ||
+          (p > rowptr[i] && colidx[p - 1] >= colidx[p]))
+        return fail(PVGLU_E_ARG,
+                    "obtain_fillin_noalloc: row %d not sorted/in range", i);
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(PVGLU_E_HIP, "obtain_fillin_noalloc: no HIP device");
+  int *base = nullptr;
+  int st = na_scratch(&base);
+  if (st)
+    return st;
+  hipStream_t s = (hipStream_t)stream;
+  int *d_rp = base, *d_rp2 = base + kNaRp, *d_cnt = base + 2 * kNaRp;
+  unsigned char *d_act = (unsigned char *)(d_cnt + kNaMaxN);
+  int *d_ci = d_cnt + kNaMaxN + kNaMaxN / 4, *d_ci2 = d_ci + kNaMaxNnz;
+  const int nnz0 = rowptr[n];
+  HIP_TRY(hipMemcpyAsync(d_rp, rowptr, sizeof(int) * (n + 1),
+                         hipMemcpyHostToDevice, s));
+  if (nnz0)
+    HIP_TRY(hipMemcpyAsync(d_ci, colidx, sizeof(int) * nnz0,
+                           hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_fill_u8, dim3((n + 255) / 256), dim3(256), 0, s, d_act,
+                     n, (unsigned char)1);
+  int nnz = nnz0;
+  for (int pass = 0; pass < n + 1; ++pass) {
+    hipLaunchKernelGGL(k_fill_rows, dim3(n), dim3(64), 0, s, 0, n, d_rp, d_ci,
+                       d_act, d_cnt, nullptr, nullptr);
+    hipLaunchKernelGGL(k_new_lengths, dim3((n + 256) / 256), dim3(256), 0, s,
+                       n, d_rp, d_cnt, d_rp2);
+    hipLaunchKernelGGL(k_scan_inclusive, dim3(1), dim3(1024), 0, s, d_rp2,
+                       n + 1);
+    HIP_TRY(hipGetLastError());
+    int nnz_new = 0;
+    HIP_TRY(hipMemcpyAsync(&nnz_new, d_rp2 + n, sizeof(int),
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (nnz_new == nnz)
+      break; // closure reached
+    if (nnz_new < nnz || nnz_new > (long)n * n)
+      return fail(PVGLU_E_HIP,
+                  "obtain_fillin_noalloc: inconsistent pass (%d -> %d)", nnz,
+                  nnz_new);
+    if (nnz_new > kNaMaxNnz)
+      return fail(PVGLU_E_ARG,
+                  "obtain_fillin_noalloc: the filled pattern reaches %d "
+                  "nonzeros, beyond the maximum of %d",
+                  nnz_new, kNaMaxNnz);
+    hipLaunchKernelGGL(k_fill_rows, dim3(n), dim3(64), 0, s, 1, n, d_rp, d_ci,
+                       d_act, d_cnt, d_rp2, d_ci2);
+    hipLaunchKernelGGL(k_next_active, dim3((n + 255) / 256), dim3(256), 0, s,
+                       n, d_rp2, d_ci2, d_cnt, d_act);
+    HIP_TRY(hipGetLastError());
+    std::swap(d_rp, d_rp2);
+    std::swap(d_ci, d_ci2);
+    nnz = nnz_new;
+  }
+  *filled_nnz = nnz;
+  if (nnz > capacity) {
+    HIP_TRY(hipStreamSynchronize(s));
+    return fail(PVGLU_E_ARG,
+                "obtain_fillin_noalloc: the filled pattern has %d nonzeros, "
+                "the output holds %d",
+                nnz, capacity);
+  }
+  HIP_TRY(hipMemcpyAsync(filled_rowptr, d_rp, sizeof(int) * (n + 1),
+                         hipMemcpyDeviceToHost, s));
+  if (nnz)
+    HIP_TRY(hipMemcpyAsync(filled_colidx, d_ci, sizeof(int) * nnz,
+                           hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
   return PVGLU_OK;
 }
