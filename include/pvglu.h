@@ -423,6 +423,90 @@ int pvglu_pv_gradients(pvglu_pv *pv, int batch, const double *g,
                        double *v_module, double *power_out, double *dP_dG,
                        double *dP_dgc, double tol, int max_iters);
 
+/* ------------------------------------------------ the reference's plant */
+
+/* The reference's own PV plant (PV/003_create_electric_system.cpp:184-382,
+ * PV/002_base_electric_system.cpp, PV/002b_sistema_electrico_base_matriz.cpp):
+ * `blocks` inverters, each a block of panels_y strings of panels_x panels
+ * (per panel an internal node, series and shunt resistances, a diode, the
+ * photocurrent and a bypass diode when `bypass`), the strings optionally
+ * cross-tied (interconnection), the block's DC side tied to ground and to an
+ * inverter: a DC voltage source of panels_x * volts_per_panel whose power is
+ * injected as an AC current (fixed from the previous iteration) between two
+ * AC nodes; AC nodes are (real, imaginary) unknown pairs, consecutive blocks'
+ * AC nodes are joined by wires and the first and last blocks' to the point
+ * of common coupling, a 25 kV AC voltage source.  Voltage sources add their
+ * currents as unknowns, with zero diagonals (hence the row swaps of
+ * reorder_matrix), and the AC ring and the parallel strings give fill-in.
+ * irradiance: W/m^2 for every panel, or per panel (panel_irradiance, block-
+ * major, then string, then position; NULL = uniform); temperature in C. */
+typedef struct pvglu_refplant pvglu_refplant;
+enum {
+  PVGLU_INTERCONNECTION_NOTHING = 0,
+  PVGLU_INTERCONNECTION_FULL = 1,
+  PVGLU_INTERCONNECTION_HONEYCOMB = 2,
+  PVGLU_INTERCONNECTION_BRIDGE = 3
+};
+int pvglu_refplant_create(int blocks, int panels_x, int panels_y,
+                          double volts_per_panel, int interconnection,
+                          double irradiance, const double *panel_irradiance,
+                          double temperature, int bypass,
+                          pvglu_refplant **out);
+void pvglu_refplant_destroy(pvglu_refplant *rp);
+enum {
+  PVGLU_REF_N = 0,         /* unknowns (nodes and voltage-source currents) */
+  PVGLU_REF_NNZ_A,         /* nonzeros of the system before fill */
+  PVGLU_REF_NONLINEAR,     /* nonlinear elements (diodes) */
+  PVGLU_REF_INVERTERS,
+  PVGLU_REF_VSOURCES,      /* voltage sources (DC inverter sides, PCC) */
+  PVGLU_REF_PANELS,
+  PVGLU_REF_PCC_NODE,      /* the PCC's AC node (1-based; imaginary = +1) */
+  PVGLU_REF_PCC_CURRENT    /* the PCC source current's unknown (real part) */
+};
+int pvglu_refplant_info(const pvglu_refplant *rp, int what);
+/* Pattern of the system at virtual conductance 1.0 (CSR, unpermuted). */
+int pvglu_refplant_pattern(const pvglu_refplant *rp, int *rowptr, int *colidx);
+/* prepare_matrices (PV/000_PVPP_simulator.cpp:10-93): reorder (AMD + the
+ * row-only zero-diagonal swaps), GPU fill-in, plan; P and Prow as
+ * reorder_matrix records them (n each).  Fails if Prow is not an
+ * involution (the Newton loop's exchange_rows(Prow) needs one). */
+int pvglu_refplant_ordering(pvglu_refplant *rp, int *P, int *Prow);
+pvglu_plan *pvglu_refplant_plan(pvglu_refplant *rp);
+/* Elements in stamping order (see pvglu_refplant.cpp for the layout);
+ * returns their count (elem / val may be NULL to count). */
+enum {
+  PVGLU_ELEM_ADMITTANCE = 0,
+  PVGLU_ELEM_AC_ADMITTANCE,
+  PVGLU_ELEM_CURRENT_SOURCE,
+  PVGLU_ELEM_VOLTAGE_SOURCE,
+  PVGLU_ELEM_NONLINEAR,
+  PVGLU_ELEM_INVERTER
+};
+int pvglu_refplant_elements(const pvglu_refplant *rp, int *elem, double *val);
+/* Unknown of each node (node_x[node - 1]); returns the node count. */
+int pvglu_refplant_node_unknowns(const pvglu_refplant *rp, int *node_x);
+int pvglu_refplant_set_newton_form(pvglu_refplant *rp, int form);
+/* The permuted system at the current iterate in the plan's CSC slot order
+ * (the values and right-hand side the refactor and solve receive). */
+int pvglu_refplant_system(pvglu_refplant *rp, double g_virtual, double *a_csc,
+                          double *b_perm);
+/* Iterate state: nonlinear element voltages and per inverter (V2 real,
+ * V2 imaginary, I1) -- the reference's vector_voltages_nonlinear_...iterate
+ * and list_values_inverter_dc_ac_iterate. */
+int pvglu_refplant_get_state(const pvglu_refplant *rp, double *v, double *inv);
+int pvglu_refplant_set_state(pvglu_refplant *rp, const double *v,
+                             const double *inv);
+/* The reference's Newton loop on this plant (PV/000_PVPP_simulator.cpp:
+ * 144-253) from the current state: ladder (or the last stage only), host
+ * assembly, permutations, GPU refactor and solve, the 011 update including
+ * the inverter refresh (PV/011_calculate_new_iteration.cpp:88-109); stops
+ * when the last stage's largest change (element voltages and inverter
+ * states) is <= tol, or at the form's noise floor.  x_out (n, may be NULL):
+ * unknowns in circuit order.  stats[0..5]: iterations, host ms, device ms,
+ * total ms, real and imaginary part of the PCC power V conj(I). */
+int pvglu_refplant_newton_dropin(pvglu_refplant *rp, double tol, int ladder,
+                                 double *x_out, double *stats);
+
 #ifdef __cplusplus
 }
 #endif

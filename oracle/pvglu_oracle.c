@@ -750,3 +750,19 @@ void orc_module_current_many(int m, const double *V, const double *Iph,
     orc_module_current(V[e], Iph[e], I0, a, Rs, Rsh[e], &I[e],
                        dIdV ? &dIdV[e] : NULL, dIdIph ? &dIdIph[e] : NULL);
 }
+
+void orc_ref_diodes(int m, const int *kind, const double *V, double T,
+                    double *f, double *df) {
+  const double q = 1.60217662E-19, k = 1.38064852E-23, na = 48.548;
+  const double I0 = 4.86E-10 * (400.0 / 175.0);
+  const double qkT = q / k / (273.15 + T);
+  for (int e = 0; e < m; e++) {
+    if (kind[e] == 0) {
+      f[e] = I0 * (exp(qkT * V[e] / na) - 1.0);
+      df[e] = I0 * qkT / na * exp(V[e] * qkT / na);
+    } else {
+      f[e] = 0.3E-6 * (exp(qkT * V[e]) - 1);
+      df[e] = qkT * 0.3E-6 * (exp(qkT * V[e]));
+    }
+  }
+}

@@ -131,6 +131,13 @@ void orc_module_current_many(int m, const double *V, const double *Iph,
                              double I0, double a, double Rs, const double *Rsh,
                              double *I, double *dIdV, double *dIdIph);
 
+/* The reference plant's diode models (PV/003_create_electric_system.cpp:
+ * 7-85): kind 0 the panel diode I0 (exp(qkT V / na) - 1), kind 1 the bypass
+ * diode 0.3e-6 (exp(qkT V) - 1), qkT = q / k / (273.15 + T); f and f' at V
+ * for m elements. */
+void orc_ref_diodes(int m, const int *kind, const double *V, double T,
+                    double *f, double *df);
+
 #ifdef __cplusplus
 }
 #endif

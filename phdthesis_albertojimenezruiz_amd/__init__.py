@@ -296,6 +296,20 @@ _PV_SIG = {
     "pvglu_pv_plan": (_VP, [_VP]),
     "pvglu_pv_newton_info": (C.c_int, [_VP, C.c_int, _VP, _VP]),
     "pvglu_pv_stamp": (C.c_int, [_VP, C.c_int, _VP, _VP, C.c_double, _VP, _VP, _VP]),
+    "pvglu_refplant_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                        C.c_double, _VP, C.c_double, C.c_int, C.POINTER(_VP)]),
+    "pvglu_refplant_destroy": (None, [_VP]),
+    "pvglu_refplant_info": (C.c_int, [_VP, C.c_int]),
+    "pvglu_refplant_pattern": (C.c_int, [_VP, _VP, _VP]),
+    "pvglu_refplant_ordering": (C.c_int, [_VP, _VP, _VP]),
+    "pvglu_refplant_plan": (_VP, [_VP]),
+    "pvglu_refplant_elements": (C.c_int, [_VP, _VP, _VP]),
+    "pvglu_refplant_node_unknowns": (C.c_int, [_VP, _VP]),
+    "pvglu_refplant_set_newton_form": (C.c_int, [_VP, C.c_int]),
+    "pvglu_refplant_system": (C.c_int, [_VP, C.c_double, _VP, _VP]),
+    "pvglu_refplant_get_state": (C.c_int, [_VP, _VP, _VP]),
+    "pvglu_refplant_set_state": (C.c_int, [_VP, _VP, _VP]),
+    "pvglu_refplant_newton_dropin": (C.c_int, [_VP, C.c_double, C.c_int, _VP, _VP]),
     "pvglu_pv_set_newton_form": (C.c_int, [_VP, C.c_int]),
     "pvglu_pv_get_newton_form": (C.c_int, [_VP]),
 }
@@ -532,3 +546,113 @@ class Multi:
     def solve_transpose(self, lu, g, y, stream=None):
         _check(lib().pvglu_multi_solve_transpose(self._h, self._arr(lu), self._arr(g),
                                                  self._arr(y), _stream(stream)))
+
+
+REF_INFO = dict(n=0, nnz_a=1, nonlinear=2, inverters=3, vsources=4, panels=5, pcc_node=6,
+                pcc_current=7)
+INTERCONNECTION = dict(nothing=0, full=1, honeycomb=2, bridge=3)
+(ELEM_ADMITTANCE, ELEM_AC_ADMITTANCE, ELEM_CURRENT_SOURCE, ELEM_VOLTAGE_SOURCE, ELEM_NONLINEAR,
+ ELEM_INVERTER) = range(6)
+
+
+class RefPlant:
+    """The reference's own plant (``pvglu_refplant_create``): blocks of
+    panels_x x panels_y panels with bypass diodes, DC/AC inverters, an AC
+    ring to the 25 kV point of common coupling."""
+
+    def __init__(self, blocks, panels_x, panels_y, volts_per_panel=23.0,
+                 interconnection="nothing", irradiance=1000.0, panel_irradiance=None,
+                 temperature=25.0, bypass=True):
+        pg = None if panel_irradiance is None else np.ascontiguousarray(panel_irradiance,
+                                                                         np.float64)
+        h = _VP()
+        _check(_pvlib().pvglu_refplant_create(
+            int(blocks), int(panels_x), int(panels_y), float(volts_per_panel),
+            INTERCONNECTION[interconnection], float(irradiance), _dp(pg), float(temperature),
+            int(bool(bypass)), C.byref(h)))
+        self._h = h
+        self.n = self.info("n")
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        self._h = None
+        if h and _lib is not None:
+            _lib.pvglu_refplant_destroy(h)
+
+    __del__ = close
+
+    def info(self, what):
+        return _check(_pvlib().pvglu_refplant_info(self._h, REF_INFO[what]))
+
+    def pattern(self):
+        rp = np.zeros(self.n + 1, np.int32)
+        ci = np.zeros(self.info("nnz_a"), np.int32)
+        _check(_pvlib().pvglu_refplant_pattern(self._h, _dp(rp), _dp(ci)))
+        return rp, ci
+
+    def ordering(self):
+        """(P, Prow) of prepare_matrices (runs the GPU fill-in and builds the plan)."""
+        P = np.zeros(self.n, np.int32)
+        Prow = np.zeros(self.n, np.int32)
+        _check(_pvlib().pvglu_refplant_ordering(self._h, _dp(P), _dp(Prow)))
+        return P, Prow
+
+    def plan(self):
+        h = _pvlib().pvglu_refplant_plan(self._h)
+        if not h:
+            msg = lib().pvglu_last_error()
+            raise PvgluError(E_ARG, msg.decode() if msg else "refplant_plan failed")
+        p = Plan.__new__(Plan)
+        p._h = None
+        p._view = C.c_void_p(h)
+        p.n = self.n
+        p._owner = self
+        return p
+
+    def elements(self):
+        k = _check(_pvlib().pvglu_refplant_elements(self._h, None, None))
+        E = np.zeros((k, 5), np.int32)
+        V = np.zeros((k, 2))
+        _check(_pvlib().pvglu_refplant_elements(self._h, _dp(E), _dp(V)))
+        return E, V
+
+    def node_unknowns(self):
+        m = _check(_pvlib().pvglu_refplant_node_unknowns(self._h, None))
+        out = np.zeros(m, np.int32)
+        _check(_pvlib().pvglu_refplant_node_unknowns(self._h, _dp(out)))
+        return out
+
+    @property
+    def newton_form(self):
+        raise AttributeError("write-only")
+
+    @newton_form.setter
+    def newton_form(self, form):
+        _check(_pvlib().pvglu_refplant_set_newton_form(self._h, int(form)))
+
+    def system(self, g_virtual=-1.0):
+        """(permuted CSC values, permuted right-hand side) at the current state."""
+        nnz = self.plan().nnz
+        a = np.zeros(nnz)
+        b = np.zeros(self.n)
+        _check(_pvlib().pvglu_refplant_system(self._h, float(g_virtual), _dp(a), _dp(b)))
+        return a, b
+
+    def get_state(self):
+        v = np.zeros(self.info("nonlinear"))
+        inv = np.zeros((self.info("inverters"), 3))
+        _check(_pvlib().pvglu_refplant_get_state(self._h, _dp(v), _dp(inv)))
+        return v, inv
+
+    def set_state(self, v=None, inv=None):
+        v = None if v is None else np.ascontiguousarray(v, np.float64)
+        inv = None if inv is None else np.ascontiguousarray(inv, np.float64)
+        _check(_pvlib().pvglu_refplant_set_state(self._h, _dp(v), _dp(inv)))
+
+    def newton_dropin(self, tol=1e-10, ladder=True):
+        """The reference's Newton loop from the current state: (x, stats, S_pcc)."""
+        x = np.zeros(self.n)
+        st = np.zeros(6)
+        _check(_pvlib().pvglu_refplant_newton_dropin(self._h, float(tol), int(bool(ladder)),
+                                                     _dp(x), _dp(st)))
+        return x, st, complex(st[4], st[5])

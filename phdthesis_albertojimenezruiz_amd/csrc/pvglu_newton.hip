@@ -1071,49 +1071,50 @@ extern "C" int pvglu_pv_gradients(pvglu_pv *pv, int batch, const double *g,
 
 // ------------------------------------------------------------- drop-in API
 
-extern "C" int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g,
-                                      double *v_module, double *x_out,
-                                      double tol, int ladder, int fixed_iters,
-                                      double *stats) {
-  begin_call();
-  if (!pv || !g || !v_module)
-    return fail(PVGLU_E_ARG, "newton_dropin: bad arguments");
-  int st = pv_prepare(pv);
-  if (st)
-    return st;
-  pvglu_plan *P = pv->plan;
-  const int n = pv->n, nnz = P->nnz, nm = pv->nmod;
-  if (!pv->dd) {
+namespace pvglu {
+
+int dropin_newton(Circuit &c, pvglu_plan *P, const std::vector<int> &Pv,
+                  const std::vector<int> &Pinv, const std::vector<int> &Prow,
+                  DropinDev *&dd, int form, double tol, int ladder,
+                  int fixed_iters, double *x_out, double *stats) {
+  (void)Pv;
+  const int n = c.unknowns(), nnz = P->nnz, nm = c.nonlinear_count();
+  if (!dd) {
     auto *d = new DropinDev();
-    pv->dd = d;
+    dd = d;
     HIP_TRY(hipMalloc(&d->a, sizeof(double) * nnz));
     HIP_TRY(hipMalloc(&d->lu, sizeof(double) * nnz));
     HIP_TRY(hipMalloc(&d->b, sizeof(double) * n));
     HIP_TRY(hipMalloc(&d->x, sizeof(double) * n));
-    HIP_TRY(hipMalloc(&d->v, sizeof(double) * nm));
+    HIP_TRY(hipMalloc(&d->v, sizeof(double) * std::max(nm, 1)));
     HIP_TRY(hipMalloc(&d->dsum, sizeof(double)));
     HIP_TRY(hipMalloc(&d->dmax, sizeof(double)));
-    HIP_TRY(hipMalloc(&d->pos, sizeof(int) * nm));
-    HIP_TRY(hipMalloc(&d->neg, sizeof(int) * nm));
+    HIP_TRY(hipMalloc(&d->pos, sizeof(int) * std::max(nm, 1)));
+    HIP_TRY(hipMalloc(&d->neg, sizeof(int) * std::max(nm, 1)));
+    // element terminals as permuted unknowns (-1: ground)
     std::vector<int> pos(nm), neg(nm);
     for (int e = 0; e < nm; e++) {
-      pos[e] = pv->mod_pos[e] >= 0 ? pv->Pinv[pv->mod_pos[e]] : -1;
-      neg[e] = pv->mod_neg[e] >= 0 ? pv->Pinv[pv->mod_neg[e]] : -1;
+      const auto t = c.nonlinear_nodes(e);
+      pos[e] = t.first ? Pinv[c.x_of(t.first)] : -1;
+      neg[e] = t.second ? Pinv[c.x_of(t.second)] : -1;
     }
-    HIP_TRY(hipMemcpy(d->pos, pos.data(), sizeof(int) * nm, hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(d->neg, neg.data(), sizeof(int) * nm, hipMemcpyHostToDevice));
+    if (nm) {
+      HIP_TRY(hipMemcpy(d->pos, pos.data(), sizeof(int) * nm, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(d->neg, neg.data(), sizeof(int) * nm, hipMemcpyHostToDevice));
+    }
   }
-  DropinDev *d = pv->dd;
-  for (int e = 0; e < nm; e++)
-    pv->mod_iph[e] = kIphPerIrradiance * g[e];
-  pv->circuit.v_iterate.assign(v_module, v_module + nm);
-  HIP_TRY(hipMemcpy(d->v, v_module, sizeof(double) * nm, hipMemcpyHostToDevice));
+  DropinDev *d = dd;
+  if (nm)
+    HIP_TRY(hipMemcpy(d->v, c.v_iterate.data(), sizeof(double) * nm,
+                      hipMemcpyHostToDevice));
+  int st;
   if ((st = pvglu_status_reset(P, nullptr)))
     return st;
-  const bool corr = pv->form == PVGLU_NEWTON_CORRECTION;
-  // correction form: node voltages of the last iteration (unpermuted; 0 at
-  // the start, so the first right-hand side is b) and their permuted copy
-  std::vector<double> vals(nnz), bvec, tmp(n), xprev(n, 0.0), xcol(n);
+  const bool corr = form == PVGLU_NEWTON_CORRECTION;
+  const bool inv = c.inverter_count() > 0;
+  // correction form: unknowns of the last iteration (circuit order; 0 at the
+  // start, so the first right-hand side is b) and their permuted copy
+  std::vector<double> vals(nnz), bvec, tmp(n), xprev(n, 0.0), xcol(n), xu(n);
   double t_host = 0, t_dev = 0;
   auto T0 = std::chrono::steady_clock::now();
   int it = 0;
@@ -1136,43 +1137,57 @@ extern "C" int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g,
       auto h0 = std::chrono::steady_clock::now();
       // create_matrix_and_vector + permutations + CSC value fill (000:185-210)
       SparseMap A;
-      pv->circuit.create_matrix_and_vector(stages[si], A, bvec);
+      c.create_matrix_and_vector(stages[si], A, bvec);
       if (corr)
-        pv->circuit.residual(stages[si], xprev, bvec);
-      A = A.interchange_row_columns(pv->Pinv.data());
-      A = A.exchange_rows(pv->Prow.data());
+        c.residual(stages[si], xprev, bvec);
+      A = A.interchange_row_columns(Pinv.data());
+      A = A.exchange_rows(Prow.data());
       for (int i = 0; i < n; i++)
-        tmp[pv->Pinv[i]] = bvec[i];
+        tmp[Pinv[i]] = bvec[i];
       for (int i = 0; i < n; i++)
-        bvec[pv->Prow[i]] = tmp[i];
-      for (int c = 0; c < n; c++)
-        for (int q = P->cp[c]; q < P->cp[c + 1]; q++)
-          vals[q] = A.D(P->ri[q], c); // fill_values_CSC_only (010:142-155)
+        bvec[Prow[i]] = tmp[i];
+      for (int col = 0; col < n; col++)
+        for (int q = P->cp[col]; q < P->cp[col + 1]; q++)
+          vals[q] = A.D(P->ri[q], col); // fill_values_CSC_only (010:142-155)
       auto h1 = std::chrono::steady_clock::now();
       HIP_TRY(hipMemcpy(d->a, vals.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
       HIP_TRY(hipMemcpy(d->b, bvec.data(), sizeof(double) * n, hipMemcpyHostToDevice));
       if ((st = pvglu_refactor(P, d->a, d->lu, 1, nullptr)) ||
           (st = pvglu_solve(P, d->lu, d->b, d->x, 1, nullptr)))
         return st;
-      if (corr) {
-        // x = x_prev + d, in unpermuted and permuted order
+      if (corr || inv)
         HIP_TRY(hipMemcpy(tmp.data(), d->x, sizeof(double) * n,
                           hipMemcpyDeviceToHost));
+      if (corr) {
+        // x = x_prev + d, in circuit and permuted order
         for (int i = 0; i < n; i++) {
-          xprev[i] = xprev[i] + tmp[pv->Pinv[i]];
-          xcol[pv->Pinv[i]] = xprev[i];
+          xprev[i] = xprev[i] + tmp[Pinv[i]];
+          xcol[Pinv[i]] = xprev[i];
         }
         HIP_TRY(hipMemcpy(d->x, xcol.data(), sizeof(double) * n,
                           hipMemcpyHostToDevice));
       }
-      if ((st = pvglu_newton_update(P, nm, 1, d->pos, d->neg, d->x, d->v, d->dsum,
-                                    d->dmax, nullptr, nullptr)))
-        return st;
       double dsum = 0, dmax = 0;
-      HIP_TRY(hipMemcpy(pv->circuit.v_iterate.data(), d->v, sizeof(double) * nm,
-                        hipMemcpyDeviceToHost));
-      HIP_TRY(hipMemcpy(&dsum, d->dsum, sizeof(double), hipMemcpyDeviceToHost));
-      HIP_TRY(hipMemcpy(&dmax, d->dmax, sizeof(double), hipMemcpyDeviceToHost));
+      if (nm) {
+        if ((st = pvglu_newton_update(P, nm, 1, d->pos, d->neg, d->x, d->v, d->dsum,
+                                      d->dmax, nullptr, nullptr)))
+          return st;
+        HIP_TRY(hipMemcpy(c.v_iterate.data(), d->v, sizeof(double) * nm,
+                          hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&dsum, d->dsum, sizeof(double), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(&dmax, d->dmax, sizeof(double), hipMemcpyDeviceToHost));
+      }
+      if (inv) {
+        // the inverters' (V2, I1) from this solution (011:88-109)
+        if (corr)
+          xu = xprev;
+        else
+          for (int i = 0; i < n; i++)
+            xu[i] = tmp[Pinv[i]];
+        double imax = 0.0;
+        dsum += c.refresh_inverters(xu, imax);
+        dmax = std::fmax(dmax, imax);
+      }
       auto h2 = std::chrono::steady_clock::now();
       t_host += std::chrono::duration<double, std::milli>(h1 - h0).count();
       t_dev += std::chrono::duration<double, std::milli>(h2 - h1).count();
@@ -1182,7 +1197,7 @@ extern "C" int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g,
       if (fixed_iters <= 0 && last) {
         if (dmax <= tol)
           break;
-        stall = dmax <= noise_of(pv->form) ? stall + 1 : 0; // FP64 noise floor
+        stall = dmax <= noise_of(form) ? stall + 1 : 0; // FP64 noise floor
         if (stall >= kStallMax)
           break;
       }
@@ -1193,11 +1208,10 @@ extern "C" int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g,
     return st;
   if (code)
     return fail(code, "drop-in Newton: status %d at %d", code, col);
-  memcpy(v_module, pv->circuit.v_iterate.data(), sizeof(double) * nm);
   if (x_out) {
     HIP_TRY(hipMemcpy(tmp.data(), d->x, sizeof(double) * n, hipMemcpyDeviceToHost));
     for (int i = 0; i < n; i++)
-      x_out[i] = tmp[pv->Pinv[i]]; // apply_permutation(x, P) (000:233)
+      x_out[i] = tmp[Pinv[i]]; // apply_permutation(x, P) (000:233)
   }
   auto T1 = std::chrono::steady_clock::now();
   if (stats) {
@@ -1206,6 +1220,30 @@ extern "C" int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g,
     stats[2] = t_dev;
     stats[3] = std::chrono::duration<double, std::milli>(T1 - T0).count();
   }
+  return PVGLU_OK;
+}
+
+} // namespace pvglu
+
+extern "C" int pvglu_pv_newton_dropin(pvglu_pv *pv, const double *g,
+                                      double *v_module, double *x_out,
+                                      double tol, int ladder, int fixed_iters,
+                                      double *stats) {
+  begin_call();
+  if (!pv || !g || !v_module)
+    return fail(PVGLU_E_ARG, "newton_dropin: bad arguments");
+  int st = pv_prepare(pv);
+  if (st)
+    return st;
+  const int nm = pv->nmod;
+  for (int e = 0; e < nm; e++)
+    pv->mod_iph[e] = kIphPerIrradiance * g[e];
+  pv->circuit.v_iterate.assign(v_module, v_module + nm);
+  if ((st = dropin_newton(pv->circuit, pv->plan, pv->P, pv->Pinv, pv->Prow,
+                          pv->dd, pv->form, tol, ladder, fixed_iters, x_out,
+                          stats)))
+    return st;
+  memcpy(v_module, pv->circuit.v_iterate.data(), sizeof(double) * nm);
   return PVGLU_OK;
 }
 

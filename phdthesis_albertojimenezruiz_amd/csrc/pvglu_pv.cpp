@@ -27,48 +27,54 @@ static double log_uniform(std::mt19937_64 &rng, double lo, double hi) {
   return std::exp(std::log(lo) + uniform01(rng()) * (std::log(hi) - std::log(lo)));
 }
 
-int pv_prepare(pvglu_pv *pv) {
-  if (pv->prepared)
-    return PVGLU_OK;
+int prepare_circuit(const Circuit &c, std::vector<int> &P,
+                    std::vector<int> &Pinv, std::vector<int> &Prow,
+                    pvglu_plan **plan) {
   // prepare_matrices (PV/000_PVPP_simulator.cpp:10-93): pattern of the
   // system assembled with virtual conductance 1.0, reordered, fill-in found,
   // symbolic plan built.
   SparseMap A;
   std::vector<double> b;
-  pv->circuit.create_matrix_and_vector(1.0, A, b);
+  c.create_matrix_and_vector(1.0, A, b);
   std::vector<int> rp, ci;
   A.csr(rp, ci);
-  const int n = pv->n;
-  pv->P.assign(n, 0);
-  pv->Prow.assign(n, 0);
-  int st = pvglu_reorder(n, rp.data(), ci.data(), pv->P.data(), pv->Prow.data());
+  const int n = c.unknowns();
+  P.assign(n, 0);
+  Prow.assign(n, 0);
+  int st = pvglu_reorder(n, rp.data(), ci.data(), P.data(), Prow.data());
   if (st)
     return st;
-  pv->Pinv.assign(n, 0);
+  Pinv.assign(n, 0);
   for (int i = 0; i < n; i++)
-    pv->Pinv[pv->P[i]] = i;
+    Pinv[P[i]] = i;
   // The reordered matrix of reorder_matrix: AMD symmetric permutation, then
   // the recorded row swaps (row k holds row Prow[k]).  The Newton loop later
   // applies exchange_rows(Prow) (new[Prow[i]] = old[i]); both agree only when
-  // Prow is an involution (true without voltage sources).
+  // Prow is an involution (disjoint swaps).
   for (int i = 0; i < n; i++)
-    if (pv->Prow[pv->Prow[i]] != i)
+    if (Prow[Prow[i]] != i)
       return fail(PVGLU_E_ARG, "prepare: row permutation is not an involution");
-  SparseMap R = A.interchange_row_columns(pv->Pinv.data());
+  SparseMap R = A.interchange_row_columns(Pinv.data());
   SparseMap R2(n, n);
   for (int k = 0; k < n; k++)
-    R2.rows[k] = R.rows[pv->Prow[k]];
+    R2.rows[k] = R.rows[Prow[k]];
   R2.csr(rp, ci);
   int *frp = nullptr, *fci = nullptr, fnnz = 0;
   st = pvglu_obtain_fillin(n, rp.data(), ci.data(), &frp, &fci, &fnnz, nullptr);
   if (st)
     return st;
-  st = pvglu_plan_create(n, frp, fci, &pv->plan);
+  st = pvglu_plan_create(n, frp, fci, plan);
   free(frp);
   free(fci);
   if (st)
     return st;
-  st = pvglu_plan_upload(pv->plan);
+  return pvglu_plan_upload(*plan);
+}
+
+int pv_prepare(pvglu_pv *pv) {
+  if (pv->prepared)
+    return PVGLU_OK;
+  int st = prepare_circuit(pv->circuit, pv->P, pv->Pinv, pv->Prow, &pv->plan);
   if (st)
     return st;
   pv->prepared = true;
