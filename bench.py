@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--schedule", default="hybrid:64:1",
                     help="variant:width:groups of the batched launch (level, component, "
                          "hybrid), or 'auto' for the autotuner's choice")
+    ap.add_argument("--profile-newton", type=int, default=0, metavar="ITERS",
+                    help="profiling mode: run ITERS replays of the batched Newton graph on "
+                         "the default workload, print their timing and exit (no headline)")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads of the all-cores CPU baseline (0: nproc)")
     return ap.parse_args()
@@ -361,6 +364,18 @@ def main():
 
     # Converged operating point of each scenario (batched device Newton)
     G = np.stack([plant.irradiance(s, 0.0) for s in range(B)], axis=1)
+    if args.profile_newton > 0:
+        # a few replays from the cold start: every instance active, the
+        # kernels of one Newton iteration as the solve runs them
+        plant.newton_batched(G, tol=1e-12, max_iters=2, ladder=False, raise_noconv=False)
+        t0 = time.perf_counter()
+        plant.newton_batched(G, tol=1e-12, max_iters=args.profile_newton, ladder=False,
+                             raise_noconv=False)
+        dt = time.perf_counter() - t0
+        print(f"[bench] profile-newton: {args.profile_newton} replays of the batched Newton "
+              f"graph ({B} x {args.inverters}x{args.strings}x{args.modules}), "
+              f"{dt / args.profile_newton * 1e3:.3f} ms per replay incl. setup", flush=True)
+        return 0
     t0 = time.perf_counter()
     state = "converged Newton state (max|dV| <= 1e-12 V, or <= 1e-11 V at the noise floor)"
     newton = None

@@ -179,21 +179,19 @@ __global__ void __launch_bounds__(256)
 
 // Correction-form right-hand side.  Per permuted row (equation of one node)
 // and instance: minus the sum of the currents leaving the node through its
-// branches, at node voltages x = xin + dx (the previous iteration's solution
-// plus its correction):
+// branches, at the node voltages x (this iteration's, k_add having formed
+// them as the previous iterate plus its correction):
 //   modules  ibr (from k_update_eval), + g_virtual (x[pos] - x[neg]) on
 //            homotopy stages;
 //   linear   l_g (x[ca] - x[cb]), cb < 0 meaning a source of EMF l_e (the
 //            Norton bus: G_N (x - 600)).
 // Each difference of node voltages is formed before it is scaled, so a
 // string's ~600 V node voltages cancel exactly and the residual carries the
-// rounding of the branch currents only.  The kernel also writes
-// xout = xin + dx for the column of the same index (every unknown once), the
-// node voltages the solve's correction applies to.  Frozen instances get a
-// zero right-hand side, so their correction is zero and xout stays put
-// after the first replay that carries their last correction over.
-// Layout: one block per kStampRows rows x up to 256 instances, as the
-// stamping gathers.
+// rounding of the branch currents only.  Frozen instances get a zero
+// right-hand side, so their correction is zero and x stays put after the
+// first replay that carries their last correction over.  Layout: one block
+// per kStampRows rows x up to 256 instances, as the stamping gathers; a
+// string row reads its two modules' branch currents only.
 __global__ void __launch_bounds__(256)
     k_residual(int n, int B, const int *__restrict__ mptr,
                const int *__restrict__ mmod, const int *__restrict__ msgn,
@@ -201,8 +199,7 @@ __global__ void __launch_bounds__(256)
                const int *__restrict__ lcb, const double *__restrict__ lg,
                const double *__restrict__ le, const int *__restrict__ pos,
                const int *__restrict__ neg, const double *__restrict__ ibr,
-               const double *__restrict__ gvp, const double *__restrict__ xin,
-               const double *__restrict__ dx, double *__restrict__ xout,
+               const double *__restrict__ gvp, const double *__restrict__ x,
                double *__restrict__ r, const int *active) {
   const int inst = (int)(blockIdx.y * blockDim.x + threadIdx.x);
   if (inst >= B)
@@ -211,10 +208,7 @@ __global__ void __launch_bounds__(256)
   const bool live = !active || active[inst];
   const double gv = live ? gvp[inst] : -1.0;
   auto node = [&](int k) {
-    if (k < 0)
-      return 0.0;
-    const size_t i = (size_t)k * B + inst;
-    return xin[i] + dx[i];
+    return k < 0 ? 0.0 : x[(size_t)k * B + inst];
   };
   int mp[kStampRows + 1], lp[kStampRows + 1];
 #pragma unroll
@@ -222,78 +216,88 @@ __global__ void __launch_bounds__(256)
     mp[q] = mptr[min(r0 + q, n)];
     lp[q] = lptr[min(r0 + q, n)];
   }
+  if (!live) {
+    for (int q = 0; q < kStampRows && r0 + q < n; ++q)
+      r[(size_t)(r0 + q) * B + inst] = 0.0;
+    return;
+  }
   bool simple = gv <= 0;
+  bool anylin = false;
 #pragma unroll
-  for (int q = 0; q < kStampRows; ++q)
+  for (int q = 0; q < kStampRows; ++q) {
     simple = simple && mp[q + 1] - mp[q] <= 2 && lp[q + 1] - lp[q] <= 1;
+    anylin = anylin || lp[q + 1] > lp[q];
+  }
   if (simple) {
     // every load first (absent terms load a valid stand-in), then combine
-    double xo[kStampRows], i0[kStampRows], i1[kStampRows], xa[kStampRows],
-        xb[kStampRows];
+    double i0[kStampRows], i1[kStampRows], xa[kStampRows], xb[kStampRows];
 #pragma unroll
     for (int q = 0; q < kStampRows; ++q) {
-      const int row = min(r0 + q, n - 1);
-      xo[q] = node(row);
       const int cm = mp[q + 1] - mp[q];
       const int m0 = cm > 0 ? mmod[mp[q]] : 0;
       const int m1 = cm > 1 ? mmod[mp[q] + 1] : m0;
       i0[q] = ibr[(size_t)m0 * B + inst];
       i1[q] = ibr[(size_t)m1 * B + inst];
-      const bool hl = lp[q + 1] > lp[q];
-      xa[q] = node(hl ? lca[lp[q]] : row);
-      const int cb = hl ? lcb[lp[q]] : row;
-      xb[q] = cb >= 0 ? node(cb) : le[lp[q]];
+      xa[q] = xb[q] = 0.0;
+    }
+    if (anylin) { // combiner and bus rows: node voltages of the linear branch
+#pragma unroll
+      for (int q = 0; q < kStampRows; ++q)
+        if (lp[q + 1] > lp[q]) {
+          xa[q] = node(lca[lp[q]]);
+          const int cb = lcb[lp[q]];
+          xb[q] = cb >= 0 ? node(cb) : le[lp[q]];
+        }
     }
 #pragma unroll
     for (int q = 0; q < kStampRows; ++q) {
       const int row = r0 + q;
       if (row >= n)
         break;
-      const size_t gi = (size_t)row * B + inst;
-      xout[gi] = xo[q];
       double acc = 0.0;
       const int cm = mp[q + 1] - mp[q];
-      if (live) {
-        if (cm > 0)
-          acc = acc - (double)msgn[mp[q]] * i0[q];
-        if (cm > 1)
-          acc = acc - (double)msgn[mp[q] + 1] * i1[q];
-        if (lp[q + 1] > lp[q])
-          acc = acc - lg[lp[q]] * (xa[q] - xb[q]);
-      }
-      r[gi] = acc;
+      if (cm > 0)
+        acc = acc - (double)msgn[mp[q]] * i0[q];
+      if (cm > 1)
+        acc = acc - (double)msgn[mp[q] + 1] * i1[q];
+      if (lp[q + 1] > lp[q])
+        acc = acc - lg[lp[q]] * (xa[q] - xb[q]);
+      r[(size_t)row * B + inst] = acc;
     }
     return;
   }
   for (int q = 0; q < kStampRows && r0 + q < n; ++q) {
     const int row = r0 + q;
-    const size_t gi = (size_t)row * B + inst;
-    xout[gi] = node(row);
     double acc = 0.0;
-    if (live) {
-      for (int k = mp[q]; k < mp[q + 1]; ++k) {
-        const int e = mmod[k];
-        double ie = ibr[(size_t)e * B + inst];
-        if (gv > 0)
-          ie = ie + gv * (node(pos[e]) - node(neg[e]));
-        acc = acc - (double)msgn[k] * ie;
-      }
-      for (int k = lp[q]; k < lp[q + 1]; ++k) {
-        const int cb = lcb[k];
-        const double xb = cb >= 0 ? node(cb) : le[k];
-        acc = acc - lg[k] * (node(lca[k]) - xb);
-      }
+    for (int k = mp[q]; k < mp[q + 1]; ++k) {
+      const int e = mmod[k];
+      double ie = ibr[(size_t)e * B + inst];
+      if (gv > 0)
+        ie = ie + gv * (node(pos[e]) - node(neg[e]));
+      acc = acc - (double)msgn[k] * ie;
     }
-    r[gi] = acc;
+    for (int k = lp[q]; k < lp[q + 1]; ++k) {
+      const int cb = lcb[k];
+      const double xb = cb >= 0 ? node(cb) : le[k];
+      acc = acc - lg[k] * (node(lca[k]) - xb);
+    }
+    r[(size_t)row * B + inst] = acc;
   }
 }
 
-// x = a + b elementwise (the correction form's final node voltages)
+// x = a + b elementwise (the correction form's node voltages: the previous
+// iterate plus its correction), two values per thread (16-byte aligned
+// buffers from hipMalloc).
 __global__ void k_add(long count, const double *__restrict__ a,
                       const double *__restrict__ b, double *__restrict__ x) {
-  const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid < count)
-    x[gid] = a[gid] + b[gid];
+  const long i = 2 * ((long)blockIdx.x * blockDim.x + threadIdx.x);
+  if (i + 1 < count) {
+    const double2 va = *(const double2 *)(a + i);
+    const double2 vb = *(const double2 *)(b + i);
+    *(double2 *)(x + i) = make_double2(va.x + vb.x, va.y + vb.y);
+  } else if (i < count) {
+    x[i] = a[i] + b[i];
+  }
 }
 
 // Stamping gathers.  A block covers kStampRows consecutive rows (slots of A,
@@ -811,10 +815,15 @@ int launch_iteration(pvglu_pv *pv, int B, int parity) {
     hipLaunchKernelGGL(k_stamp_a, ga.grid, ga.block, 0, d->s, nnz, B,
                        d->base_a, d->a_ptr, d->a_mod, d->a_sgn, d->geq, d->gvs,
                        d->A, d->active);
+    // x of this iteration = the previous one plus its correction (every
+    // instance: a frozen one's correction is zero after its first replay)
+    const long cnt = (long)n * B;
+    hipLaunchKernelGGL(k_add, grid_for((cnt + 1) / 2), dim3(256), 0, d->s, cnt,
+                       d->X[parity], d->dx, d->X[1 - parity]);
     hipLaunchKernelGGL(k_residual, gb.grid, gb.block, 0, d->s, n, B, d->b_ptr,
                        d->b_mod, d->b_sgn, d->l_ptr, d->l_ca, d->l_cb, d->l_g,
-                       d->l_e, d->pos, d->neg, d->ieq, d->gvs, d->X[parity],
-                       d->dx, d->X[1 - parity], d->b, d->active);
+                       d->l_e, d->pos, d->neg, d->ieq, d->gvs, d->X[1 - parity],
+                       d->b, d->active);
     HIP_TRY(hipGetLastError());
     if ((st = pvglu_refactor_solve(pv->plan, d->A, d->LU, d->b, d->dx, B, d->s)))
       return st;
@@ -953,7 +962,7 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
     // the last replay wrote X[parity]; its correction is still in dx (zero
     // for the instances frozen before it)
     const long cnt = (long)pv->n * B;
-    hipLaunchKernelGGL(k_add, grid_for(cnt), dim3(256), 0, d->s, cnt,
+    hipLaunchKernelGGL(k_add, grid_for((cnt + 1) / 2), dim3(256), 0, d->s, cnt,
                        d->X[parity], d->dx, d->x);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(d->s));
