@@ -383,6 +383,20 @@ enum { PVGLU_NEWTON_CORRECTION = 0, PVGLU_NEWTON_DIRECT = 1 };
 int pvglu_pv_set_newton_form(pvglu_pv *pv, int form);
 int pvglu_pv_get_newton_form(const pvglu_pv *pv);
 
+/* Batched Newton (pvglu_pv_newton_batched, pvglu_pv_gradients): on = 1 (the
+ * default) stops spending work on finished instances (converged, or stopped
+ * at the noise floor).  Between polls (every 4 graph replays) the refactor +
+ * solve skips every instance tile (the schedule's width of consecutive
+ * instances) with no active instance; at a poll that finds at most three
+ * quarters of the working batch active, the finished instances' results are
+ * written to their columns of the caller's arrays and the active ones are
+ * gathered into a smaller working batch, for which the iteration graphs are
+ * captured again.  Per-instance arithmetic does not depend on the batch, so
+ * results are bit-identical with 0, which runs every instance on every
+ * iteration (for measurement).  No counterpart in the reference (one
+ * instance per solve). */
+int pvglu_pv_set_skip_frozen(pvglu_pv *pv, int on);
+
 /* Host drop-in Newton solve of one plant, with the reference's call sequence
  * (PV/000_PVPP_simulator.cpp:144-235): symbolic setup once, then per
  * iteration host assembly, permutation, CSC value fill, GPU refactor, GPU

@@ -312,6 +312,7 @@ _PV_SIG = {
     "pvglu_refplant_newton_dropin": (C.c_int, [_VP, C.c_double, C.c_int, _VP, _VP]),
     "pvglu_pv_set_newton_form": (C.c_int, [_VP, C.c_int]),
     "pvglu_pv_get_newton_form": (C.c_int, [_VP]),
+    "pvglu_pv_set_skip_frozen": (C.c_int, [_VP, C.c_int]),
 }
 
 
@@ -451,6 +452,12 @@ class PV:
     @newton_form.setter
     def newton_form(self, form):
         _check(_pvlib().pvglu_pv_set_newton_form(self._h, int(form)))
+
+    def set_skip_frozen(self, on=True):
+        """Batched Newton: skip the refactor + solve of instance tiles whose
+        instances have all finished (default on; off runs every instance on
+        every iteration, with bit-identical results)."""
+        _check(_pvlib().pvglu_pv_set_skip_frozen(self._h, 1 if on else 0))
 
     def v0(self, batch=None):
         v = np.full(self.nmod, 600.0 / self.M)

@@ -60,6 +60,9 @@ struct DevicePlan {
       *ut_fast = nullptr, *lt_rec = nullptr, *lt_seg = nullptr,
       *lt_run = nullptr, *lt_fast = nullptr, *drf = nullptr;
   int *status = nullptr; // [code, column, instance, pad]
+  // instance activity flags of the launch being issued (masked launches
+  // only; null otherwise)
+  const int *act = nullptr;
   std::vector<void *> allocs;
   // component groups (first component of each group, 1 / 2 / 4 groups) and
   // the side streams and events of the grouped schedules
@@ -79,7 +82,7 @@ static Pat pat_of(const DevicePlan *d) {
              d->lu_fast,   d->u_rec,    d->u_seg,  d->u_run,     d->u_fast,
              d->l_rec,     d->l_seg,    d->l_run,  d->l_fast,    d->ut_rec,
              d->ut_seg,    d->ut_run,   d->ut_fast, d->lt_rec,   d->lt_seg,
-             d->lt_run,    d->lt_fast,  d->drf,    d->status};
+             d->lt_run,    d->lt_fast,  d->drf,    d->status, d->act};
 }
 static Lev lev_of(const DevLevels &L) {
   return Lev{L.level_chain_ptr, L.chain_col_ptr, L.cols,
@@ -335,7 +338,7 @@ static bool valid_width(int w) {
 }
 
 static void pick_default_schedule(pvglu_plan *P, int B, int &variant, int &W,
-                                  int &threads) {
+                                  int &threads, bool fit = false) {
   variant = P->sched_variant;
   W = P->sched_width;
   // measured on MI355X: HYBRID wins from batch 16 up (40x24x28 plant:
@@ -343,6 +346,11 @@ static void pick_default_schedule(pvglu_plan *P, int B, int &variant, int &W,
   // 0.282 ms), COMPONENT below (batch 1: 0.050 vs 0.091 ms)
   if (!valid_width(W))
     W = B >= 64 ? 64 : (B >= 32 ? 32 : (B >= 16 ? 16 : (B >= 4 ? 4 : 1)));
+  // fit: a width chosen for a larger batch narrowed to this one (a batch
+  // that shrank, such as the batched Newton's after compaction)
+  if (fit)
+    while (W > 1 && W > B)
+      W >>= 1;
   if (variant == PVGLU_SCHED_AUTO)
     variant = (B < 16 && P->ncomp >= 8) ? PVGLU_SCHED_COMPONENT
                                         : PVGLU_SCHED_HYBRID;
@@ -588,13 +596,21 @@ static int launch_component(pvglu_plan *P, int W, int mode, int threads,
   }
 }
 
-// mode: 0 refactor, 1 refactor+solve, 2 solve, 3 transposed solve
+// mode: 0 refactor, 1 refactor+solve, 2 solve, 3 transposed solve.  act:
+// optional device flags per instance; tiles without an active instance are
+// skipped (their factors and solution keep their earlier values).
 static int run_numeric(pvglu_plan *P, int mode, const double *A, double *LU,
-                       const double *b, double *x, int B, void *stream) {
+                       const double *b, double *x, int B, void *stream,
+                       const int *act = nullptr, bool fit = false) {
   begin_call();
   int st = require_device(P);
   if (st)
     return st;
+  struct MaskScope { // the flags apply to this call's launches only
+    DevicePlan *d;
+    MaskScope(DevicePlan *d, const int *a) : d(d) { d->act = a; }
+    ~MaskScope() { d->act = nullptr; }
+  } scope(P->dev, act);
   if (B <= 0)
     return fail(PVGLU_E_ARG, "batch must be positive");
   if ((mode != 2 && mode != 3 && (!A || !LU)) ||
@@ -602,7 +618,7 @@ static int run_numeric(pvglu_plan *P, int mode, const double *A, double *LU,
     return fail(PVGLU_E_ARG, "null device pointer");
   hipStream_t s = (hipStream_t)stream;
   int variant, W, threads;
-  pick_default_schedule(P, B, variant, W, threads);
+  pick_default_schedule(P, B, variant, W, threads, fit);
   if (variant == PVGLU_SCHED_COMPONENT)
     return launch_component(P, W, mode, threads, A, LU, b, x, B, s);
   return launch_steps(P, W, mode, variant == PVGLU_SCHED_HYBRID,
@@ -624,6 +640,14 @@ extern "C" int pvglu_refactor_solve(pvglu_plan *P, const double *a, double *lu,
                                     void *stream) {
   return run_numeric(P, 1, a, lu, b, x, batch, stream);
 }
+
+namespace pvglu {
+int refactor_solve_masked(pvglu_plan *P, const double *a, double *lu,
+                          const double *b, double *x, int batch, void *stream,
+                          const int *active) {
+  return run_numeric(P, 1, a, lu, b, x, batch, stream, active, true);
+}
+} // namespace pvglu
 
 extern "C" int pvglu_solve_transpose(pvglu_plan *P, const double *lu,
                                      const double *g, double *y, int batch,

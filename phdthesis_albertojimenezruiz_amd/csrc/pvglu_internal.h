@@ -25,6 +25,15 @@ void clear_error();
 // their own.  Defined in pvglu_device.hip.
 void begin_call();
 
+// pvglu_refactor_solve whose launches skip every instance tile with no
+// active instance (active: device flags, one per instance, or null; the
+// batched Newton's frozen instances), with the schedule's width narrowed to
+// the batch when it is wider (a batch compacted below the width it was
+// tuned for).  Defined in pvglu_device.hip.
+int refactor_solve_masked(pvglu_plan *P, const double *a, double *lu,
+                          const double *b, double *x, int batch, void *stream,
+                          const int *active);
+
 // Levels of chains of columns in the flattened form of
 // PV/008_convert_chains_columns_levels_into_vector.cpp:4-30.
 struct LevelSet {

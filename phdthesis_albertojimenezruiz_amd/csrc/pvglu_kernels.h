@@ -37,6 +37,23 @@ template <int W> __device__ __forceinline__ int uniform(int v) {
     return v;
 }
 
+// True when no instance of the calling lane's W-lane tile is active (act
+// non-null).  The W lanes of a tile are aligned consecutive lanes of one
+// wavefront (64 % W == 0); `valid` is false for lanes past the batch.  In
+// the block-per-tile kernels every group of W lanes holds the same tile, so
+// the answer is the same in every wavefront of the block.
+template <int W>
+__device__ __forceinline__ bool tile_idle(const int *act, int inst,
+                                          bool valid) {
+  const unsigned long long m = __ballot(valid && act[inst] != 0);
+  if constexpr (W == 64) {
+    return m == 0;
+  } else {
+    const int first = (int)(threadIdx.x & 63) / W * W;
+    return ((m >> first) & ((1ull << W) - 1)) == 0;
+  }
+}
+
 
 // One column of the refactor for one instance (gather form), optionally
 // followed by the forward substitution of row c.
@@ -981,6 +998,8 @@ __global__ void __launch_bounds__(256)
   const int inst = tile * W + lane;
   if (inst >= B)
     return;
+  if (P.act && tile_idle<W>(P.act, inst, true))
+    return;
   run_chain<W, OP, 0, STREAM>(P, L, chain, A, LU, b, x, (size_t)B, inst);
 }
 
@@ -1043,6 +1062,8 @@ __device__ __forceinline__ void component_body(const Pat &P, const Lev &Llu,
   const int inst = tile * W + lane;
   const bool active = inst < B;
   const size_t Bs = (size_t)B;
+  if (P.act && tile_idle<W>(P.act, inst, active))
+    return; // the whole block: one tile, before any barrier
   if (mode == 3) {
     component_levels<W, 0>(P, Llu, comp, slot, slots, active, A, LU, b, x,
                                Bs, inst, OP_UT);
@@ -1090,6 +1111,8 @@ __global__ void __launch_bounds__(256)
   const int slot = uniform<W>((int)threadIdx.x / W);
   const int inst = tile * W + (int)threadIdx.x % W;
   const bool active = inst < B;
+  if (P.act && tile_idle<W>(P.act, inst, active))
+    return; // the whole block: one tile, before any barrier
   // level set k selected field by field (no indexed array of kernel
   // arguments, which would live in scratch memory)
   for (int k = 0; k < 4; ++k) {

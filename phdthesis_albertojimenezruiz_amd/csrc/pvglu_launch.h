@@ -49,6 +49,11 @@ struct Pat {
   const int *__restrict__ lt_fast;
   const int *__restrict__ drf;
   int *status;
+  // Optional per-instance activity flags (null: every instance runs).  A
+  // W-instance tile none of whose instances is active skips the launch's
+  // work: the batched Newton's frozen instances, whose system and solution
+  // no longer change.
+  const int *act;
 };
 
 struct Lev {

@@ -87,6 +87,7 @@ struct NewtonDev {
   int nstages = 12; // stages of the current solve (12 = ladder, 1 = last only)
   double tol = 1e-10;    // captured into the graph; re-captured when changed
   int form = PVGLU_NEWTON_CORRECTION; // form the graphs were captured for
+  int skip = 1; // frozen-tile skipping the graphs were captured with
   hipStream_t s = nullptr;
   // one graph per iteration; the correction form alternates two (parity p
   // reads node voltages X[p] and writes X[1 - p])
@@ -94,6 +95,19 @@ struct NewtonDev {
   hipGraph_t graph = nullptr, graph2 = nullptr;
   std::vector<void *> allocs;
   std::vector<void *> inst_allocs;
+  // compaction of the batched Newton: two alternate sets of the per-instance
+  // state the iteration carries over (allocated at the first compaction of a
+  // batch size) and index lists [3 * B] (survivors, leavers, their home
+  // instances)
+  struct InstSet {
+    double *G = nullptr, *v = nullptr, *geq = nullptr, *ieq = nullptr,
+           *X0 = nullptr, *X1 = nullptr, *dx = nullptr, *x = nullptr,
+           *gvs = nullptr, *lastmax = nullptr;
+    int *stage = nullptr, *stall = nullptr, *iters = nullptr;
+  };
+  InstSet alt[2];
+  int *cidx = nullptr;
+  std::vector<void *> alt_allocs;
 };
 
 void newton_dev_destroy(NewtonDev *d) {
@@ -110,6 +124,8 @@ void newton_dev_destroy(NewtonDev *d) {
   for (void *p : d->allocs)
     (void)hipFree(p);
   for (void *p : d->inst_allocs)
+    (void)hipFree(p);
+  for (void *p : d->alt_allocs)
     (void)hipFree(p);
   delete d;
 }
@@ -286,17 +302,41 @@ __global__ void __launch_bounds__(256)
 }
 
 // x = a + b elementwise (the correction form's node voltages: the previous
-// iterate plus its correction), two values per thread (16-byte aligned
-// buffers from hipMalloc).
+// iterate plus its correction), four values per thread as two 16-byte loads
+// of each operand (buffers from hipMalloc, 256-byte aligned).  With `active`
+// (flags per instance, values [row][instance] with B instances), the
+// correction b of a frozen instance is cleared once added: a frozen
+// instance's right-hand side is zero, so its correction is zero from then
+// on, and a refactor + solve that skips a tile of frozen instances must find
+// it so.
 __global__ void k_add(long count, const double *__restrict__ a,
-                      const double *__restrict__ b, double *__restrict__ x) {
-  const long i = 2 * ((long)blockIdx.x * blockDim.x + threadIdx.x);
-  if (i + 1 < count) {
-    const double2 va = *(const double2 *)(a + i);
-    const double2 vb = *(const double2 *)(b + i);
-    *(double2 *)(x + i) = make_double2(va.x + vb.x, va.y + vb.y);
-  } else if (i < count) {
-    x[i] = a[i] + b[i];
+                      double *__restrict__ b, double *__restrict__ x, int B,
+                      const int *__restrict__ active) {
+  const long i = 4 * ((long)blockIdx.x * blockDim.x + threadIdx.x);
+  if (i + 3 < count) {
+    // streaming hints: the operands and the sum are next read after the
+    // refactor + solve has streamed far more than the caches hold
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 a0 = __builtin_nontemporal_load((const d2 *)(a + i));
+    const d2 a1 = __builtin_nontemporal_load((const d2 *)(a + i + 2));
+    const d2 b0 = __builtin_nontemporal_load((const d2 *)(b + i));
+    const d2 b1 = __builtin_nontemporal_load((const d2 *)(b + i + 2));
+    __builtin_nontemporal_store(a0 + b0, (d2 *)(x + i));
+    __builtin_nontemporal_store(a1 + b1, (d2 *)(x + i + 2));
+    if (active) {
+      int k = (int)(i % B);
+      for (int j = 0; j < 4; ++j) {
+        if (!active[k])
+          b[i + j] = 0.0;
+        k = k + 1 == B ? 0 : k + 1;
+      }
+    }
+  } else {
+    for (long j = i; j < count; ++j) {
+      x[j] = a[j] + b[j];
+      if (active && !active[(int)(j % B)])
+        b[j] = 0.0;
+    }
   }
 }
 
@@ -417,6 +457,38 @@ __global__ void k_zero(long count, double *p) {
   const long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (gid < count)
     p[gid] = 0.0;
+}
+
+// Compaction of a [row][instance] array (rows x Bc, instance-minor) to the
+// instances sel[0..Bn) (rows x Bn): out[r][j] = in[r][sel[j]] (+ add[r][sel[j]]
+// when add is given: the correction form's pending correction folded into the
+// node voltages).
+template <typename T>
+__global__ void k_gather(long rows, int Bc, int Bn, const int *__restrict__ sel,
+                         const T *__restrict__ in, const T *__restrict__ add,
+                         T *__restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * Bn)
+    return;
+  const long r = i / Bn;
+  const long src = r * Bc + sel[i - r * Bn];
+  out[i] = add ? (T)(in[src] + add[src]) : in[src];
+}
+
+// The inverse for the instances leaving a compacted batch: out[r][to[k]] =
+// in[r][from[k]] (+ add), out with Bh instances per row (the caller's batch).
+template <typename T>
+__global__ void k_scatter(long rows, int Bc, int Bh, int cnt,
+                          const int *__restrict__ from,
+                          const int *__restrict__ to, const T *__restrict__ in,
+                          const T *__restrict__ add, T *__restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * cnt)
+    return;
+  const long r = i / cnt;
+  const long k = i - r * cnt;
+  const long src = r * Bc + from[k];
+  out[r * Bh + to[k]] = add ? (T)(in[src] + add[src]) : in[src];
 }
 
 __global__ void k_power_grad_bus(int nbus, int B, const int *__restrict__ bus_c,
@@ -732,6 +804,11 @@ int newton_alloc_instances(pvglu_pv *pv, int B) {
   for (void *p : d->inst_allocs)
     (void)hipFree(p);
   d->inst_allocs.clear();
+  for (void *p : d->alt_allocs)
+    (void)hipFree(p);
+  d->alt_allocs.clear();
+  d->alt[0] = d->alt[1] = NewtonDev::InstSet{};
+  d->cidx = nullptr;
   d->B = 0;
   const size_t nm = pv->nmod, n = pv->n, nnz = pv->plan->nnz, nc = pv->ncab;
   auto &I = d->inst_allocs;
@@ -808,7 +885,10 @@ int launch_iteration(pvglu_pv *pv, int B, int parity) {
   if (!corr) {
     if ((st = launch_stamp(pv, B, d->active, false)))
       return st;
-    if ((st = pvglu_refactor_solve(pv->plan, d->A, d->LU, d->b, d->x, B, d->s)))
+    // frozen instances: A and b unchanged, so their solution stays put; a
+    // tile of them is skipped
+    if ((st = refactor_solve_masked(pv->plan, d->A, d->LU, d->b, d->x, B, d->s,
+                                    d->skip ? d->active : nullptr)))
       return st;
   } else {
     const RowGrid ga = stamp_grid(nnz, B), gb = stamp_grid(n, B);
@@ -818,14 +898,17 @@ int launch_iteration(pvglu_pv *pv, int B, int parity) {
     // x of this iteration = the previous one plus its correction (every
     // instance: a frozen one's correction is zero after its first replay)
     const long cnt = (long)n * B;
-    hipLaunchKernelGGL(k_add, grid_for((cnt + 1) / 2), dim3(256), 0, d->s, cnt,
-                       d->X[parity], d->dx, d->X[1 - parity]);
+    hipLaunchKernelGGL(k_add, grid_for((cnt + 3) / 4), dim3(256), 0, d->s, cnt,
+                       d->X[parity], d->dx, d->X[1 - parity], B,
+                       d->skip ? d->active : nullptr);
     hipLaunchKernelGGL(k_residual, gb.grid, gb.block, 0, d->s, n, B, d->b_ptr,
                        d->b_mod, d->b_sgn, d->l_ptr, d->l_ca, d->l_cb, d->l_g,
                        d->l_e, d->pos, d->neg, d->ieq, d->gvs, d->X[1 - parity],
                        d->b, d->active);
     HIP_TRY(hipGetLastError());
-    if ((st = pvglu_refactor_solve(pv->plan, d->A, d->LU, d->b, d->dx, B, d->s)))
+    // a tile of frozen instances is skipped: k_add cleared their corrections
+    if ((st = refactor_solve_masked(pv->plan, d->A, d->LU, d->b, d->dx, B,
+                                    d->s, d->skip ? d->active : nullptr)))
       return st;
     xnew = d->X[1 - parity];
     dx = d->dx;
@@ -885,6 +968,213 @@ int build_graph(pvglu_pv *pv, int B) {
 
 } // namespace
 
+// --------------------------------------------------- batch compaction
+//
+// Instances finish their Newton solves at different iterations (the cfg2
+// cold start: 248 to 484 iterations, median 375).  When at a poll at most
+// three quarters of the working batch is still active, the finished
+// instances' results go to their columns of the caller's arrays (the "home"
+// set: G, v, x, iters, lastmax at batch B) and the active ones are gathered
+// into a denser working batch in one of two alternate sets, the
+// correction form's pending correction folded into the node voltages
+// (X <- X + dx, dx <- 0: the same sum the next replay would form).  The
+// graphs are captured again for the smaller batch.  Every kernel's
+// arithmetic per instance is independent of the batch width and of the
+// other instances, so results are bit-identical with or without compaction.
+
+namespace {
+
+using InstSet = NewtonDev::InstSet;
+
+__global__ void k_fill_i32(int n, int *p, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n)
+    p[i] = v;
+}
+
+InstSet work_set(const NewtonDev *d) {
+  InstSet w;
+  w.G = d->G, w.v = d->v, w.geq = d->geq, w.ieq = d->ieq;
+  w.X0 = d->X[0], w.X1 = d->X[1], w.dx = d->dx, w.x = d->x;
+  w.gvs = d->gvs, w.lastmax = d->lastmax;
+  w.stage = d->stage, w.stall = d->stall, w.iters = d->iters;
+  return w;
+}
+
+void set_work(NewtonDev *d, const InstSet &w) {
+  d->G = w.G, d->v = w.v, d->geq = w.geq, d->ieq = w.ieq;
+  d->X[0] = w.X0, d->X[1] = w.X1, d->dx = w.dx, d->x = w.x;
+  d->gvs = w.gvs, d->lastmax = w.lastmax;
+  d->stage = w.stage, d->stall = w.stall, d->iters = w.iters;
+}
+
+int alloc_alt(pvglu_pv *pv) {
+  NewtonDev *d = pv->nd;
+  if (d->cidx)
+    return PVGLU_OK;
+  const size_t nm = pv->nmod, n = pv->n, B = d->B;
+  auto &I = d->alt_allocs;
+  int st;
+  for (InstSet &a : d->alt)
+    if ((st = dalloc(I, &a.G, nm * B)) || (st = dalloc(I, &a.v, nm * B)) ||
+        (st = dalloc(I, &a.geq, nm * B)) || (st = dalloc(I, &a.ieq, nm * B)) ||
+        (st = dalloc(I, &a.X0, n * B)) || (st = dalloc(I, &a.X1, n * B)) ||
+        (st = dalloc(I, &a.dx, n * B)) || (st = dalloc(I, &a.x, n * B)) ||
+        (st = dalloc(I, &a.gvs, B)) || (st = dalloc(I, &a.lastmax, B)) ||
+        (st = dalloc(I, &a.stage, B)) || (st = dalloc(I, &a.stall, B)) ||
+        (st = dalloc(I, &a.iters, B)))
+      return st;
+  return dalloc(I, &d->cidx, 3 * B);
+}
+
+template <typename T>
+void gather(hipStream_t s, long rows, int Bc, int Bn, const int *sel,
+            const T *in, const T *add, T *out) {
+  const long cnt = rows * Bn;
+  if (cnt > 0)
+    hipLaunchKernelGGL(k_gather<T>, dim3((unsigned)((cnt + 255) / 256)),
+                       dim3(256), 0, s, rows, Bc, Bn, sel, in, add, out);
+}
+
+template <typename T>
+void scatter(hipStream_t s, long rows, int Bc, int Bh, int cnt, const int *from,
+             const int *to, const T *in, const T *add, T *out) {
+  const long tot = rows * cnt;
+  if (tot > 0)
+    hipLaunchKernelGGL(k_scatter<T>, dim3((unsigned)((tot + 255) / 256)),
+                       dim3(256), 0, s, rows, Bc, Bh, cnt, from, to, in, add,
+                       out);
+}
+
+struct Compaction {
+  pvglu_pv *pv;
+  InstSet home; // the caller's batch, in the caller's instance order
+  int B, Bc, next = 0;
+  std::vector<int> map; // working instance -> caller instance
+  bool active = false;  // the working batch is not the home set
+  Compaction(pvglu_pv *pv, int B)
+      : pv(pv), home(work_set(pv->nd)), B(B), Bc(B), map(B) {
+    for (int k = 0; k < B; ++k)
+      map[k] = k;
+  }
+  // on every exit: the home set back in place (after finish() it holds every
+  // instance's result; on an error path it is incomplete) and the graphs of
+  // the compacted batch dropped
+  ~Compaction() {
+    if (active) {
+      set_work(pv->nd, home);
+      drop_graphs(pv->nd);
+    }
+  }
+  int caller_instance(int k) const { return k >= 0 && k < Bc ? map[k] : k; }
+
+  // Results of the working instances from[0..cnt) (device list) to the home
+  // columns to[0..cnt).  parity: the node-voltage buffer the next replay
+  // would read.
+  void results_home(int cnt, const int *from, const int *to, int parity,
+                    bool corr) {
+    NewtonDev *d = pv->nd;
+    const InstSet w = work_set(d);
+    scatter<double>(d->s, pv->nmod, Bc, B, cnt, from, to, w.v, nullptr, home.v);
+    if (corr)
+      scatter<double>(d->s, pv->n, Bc, B, cnt, from, to, parity ? w.X1 : w.X0,
+                      w.dx, home.x);
+    else
+      scatter<double>(d->s, pv->n, Bc, B, cnt, from, to, w.x, nullptr, home.x);
+    scatter<int>(d->s, 1, Bc, B, cnt, from, to, w.iters, nullptr, home.iters);
+    scatter<double>(d->s, 1, Bc, B, cnt, from, to, w.lastmax, nullptr,
+                    home.lastmax);
+  }
+
+  int compact(int parity, bool corr) {
+    NewtonDev *d = pv->nd;
+    int code = 0, col = 0, inst = 0, st;
+    if ((st = pvglu_status(pv->plan, &code, &col, &inst)))
+      return st;
+    if (code) // stop at the first recorded error
+      return fail(code, "batched Newton: status %d at column/element %d, instance %d",
+                  code, col, caller_instance(inst));
+    std::vector<int> act(Bc);
+    HIP_TRY(hipMemcpy(act.data(), d->active, sizeof(int) * Bc,
+                      hipMemcpyDeviceToHost));
+    std::vector<int> idx, from, to; // idx: survivors, then leavers
+    for (int k = 0; k < Bc; ++k)
+      if (act[k])
+        idx.push_back(k);
+    const int Bn = (int)idx.size(), L = Bc - Bn;
+    if (Bn == 0 || L == 0)
+      return PVGLU_OK;
+    for (int k = 0; k < Bc; ++k)
+      if (!act[k]) {
+        from.push_back(k);
+        to.push_back(map[k]);
+      }
+    idx.insert(idx.end(), from.begin(), from.end());
+    idx.insert(idx.end(), to.begin(), to.end());
+    if ((st = alloc_alt(pv)))
+      return st;
+    HIP_TRY(hipMemcpy(d->cidx, idx.data(), sizeof(int) * idx.size(),
+                      hipMemcpyHostToDevice));
+    const int *sel = d->cidx, *dfrom = d->cidx + Bn, *dto = d->cidx + Bn + L;
+    results_home(L, dfrom, dto, parity, corr);
+    const InstSet w = work_set(d);
+    InstSet nx = d->alt[next];
+    const long nm = pv->nmod, n = pv->n;
+    gather<double>(d->s, nm, Bc, Bn, sel, w.G, nullptr, nx.G);
+    gather<double>(d->s, nm, Bc, Bn, sel, w.v, nullptr, nx.v);
+    gather<double>(d->s, nm, Bc, Bn, sel, w.geq, nullptr, nx.geq);
+    gather<double>(d->s, nm, Bc, Bn, sel, w.ieq, nullptr, nx.ieq);
+    if (corr) {
+      gather<double>(d->s, n, Bc, Bn, sel, parity ? w.X1 : w.X0, w.dx,
+                     parity ? nx.X1 : nx.X0);
+      hipLaunchKernelGGL(k_zero, grid_for(n * Bn), dim3(256), 0, d->s,
+                         n * Bn, nx.dx);
+    } // direct form: the next replay's solve writes every x
+    gather<double>(d->s, 1, Bc, Bn, sel, w.gvs, nullptr, nx.gvs);
+    gather<double>(d->s, 1, Bc, Bn, sel, w.lastmax, nullptr, nx.lastmax);
+    gather<int>(d->s, 1, Bc, Bn, sel, w.stage, nullptr, nx.stage);
+    gather<int>(d->s, 1, Bc, Bn, sel, w.stall, nullptr, nx.stall);
+    gather<int>(d->s, 1, Bc, Bn, sel, w.iters, nullptr, nx.iters);
+    hipLaunchKernelGGL(k_fill_i32, grid_for(Bn), dim3(256), 0, d->s, Bn,
+                       d->active, 1);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(d->s));
+    // the home set's x keeps the caller's results; the working set brings
+    // its own buffer (written by the direct form's solves)
+    set_work(d, nx);
+    next ^= 1;
+    std::vector<int> nmap(Bn);
+    for (int j = 0; j < Bn; ++j)
+      nmap[j] = map[idx[j]];
+    map.swap(nmap);
+    Bc = Bn;
+    active = true;
+    drop_graphs(d);
+    return build_graph(pv, Bc);
+  }
+
+  // every working instance's results home; the home set back in place
+  int finish(int parity, bool corr) {
+    NewtonDev *d = pv->nd;
+    std::vector<int> idx(2 * Bc);
+    for (int k = 0; k < Bc; ++k) {
+      idx[k] = k;
+      idx[Bc + k] = map[k];
+    }
+    HIP_TRY(hipMemcpy(d->cidx, idx.data(), sizeof(int) * idx.size(),
+                      hipMemcpyHostToDevice));
+    results_home(Bc, d->cidx, d->cidx + Bc, parity, corr);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(d->s));
+    set_work(d, home);
+    drop_graphs(d);
+    active = false;
+    return PVGLU_OK;
+  }
+};
+
+} // namespace
+
 // ------------------------------------------------------------- batched API
 
 static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
@@ -908,11 +1198,13 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
   if ((st = pvglu_status_reset(pv->plan, d->s)))
     return st;
   const int nst = ladder ? 12 : 1;
-  if (d->exec && (d->nstages != nst || d->tol != tol || d->form != pv->form))
+  if (d->exec && (d->nstages != nst || d->tol != tol || d->form != pv->form ||
+                  d->skip != pv->skip_frozen))
     drop_graphs(d);
   d->nstages = nst;
   d->tol = tol;
   d->form = pv->form;
+  d->skip = pv->skip_frozen;
   const bool corr = d->form == PVGLU_NEWTON_CORRECTION;
   hipLaunchKernelGGL(k_newton_init, grid_for(B), dim3(256), 0, d->s, B,
                      kLadder[12 - nst], d->stage, d->gvs, d->active, d->stall,
@@ -936,12 +1228,17 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
   // are zero), so the extra replays of a burst cost time only, and the GPU
   // no longer idles for a host round trip after each iteration.  The
   // correction form alternates its two graphs (replay j reads X[j % 2]).
+  // With skip_frozen, a poll that finds at most three quarters of the
+  // working batch active compacts it (Compaction above).
   constexpr int kPoll = 4;
   int iters = 0, remaining = 1, parity = 0;
+  Compaction cm(pv, B);
   auto t0 = std::chrono::steady_clock::now();
   while (remaining > 0) {
     if (iters >= max_iters) {
       // hand back the current iterate for diagnosis
+      if (cm.active && (st = cm.finish(parity, corr)))
+        return st;
       (void)hipMemcpy(v_module, d->v, sizeof(double) * nm * B,
                       hipMemcpyDeviceToHost);
       return fail(PVGLU_E_NOCONV, "batched Newton: %d iterations, %d instances left",
@@ -957,13 +1254,19 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
     HIP_TRY(hipMemcpyAsync(&remaining, d->remaining, sizeof(int),
                            hipMemcpyDeviceToHost, d->s));
     HIP_TRY(hipStreamSynchronize(d->s));
+    if (pv->skip_frozen && remaining > 0 && 4 * remaining <= 3 * cm.Bc)
+      if ((st = cm.compact(parity, corr)))
+        return st;
   }
-  if (corr) {
+  if (cm.active) {
+    if ((st = cm.finish(parity, corr)))
+      return st;
+  } else if (corr) {
     // the last replay wrote X[parity]; its correction is still in dx (zero
     // for the instances frozen before it)
     const long cnt = (long)pv->n * B;
-    hipLaunchKernelGGL(k_add, grid_for((cnt + 1) / 2), dim3(256), 0, d->s, cnt,
-                       d->X[parity], d->dx, d->x);
+    hipLaunchKernelGGL(k_add, grid_for((cnt + 3) / 4), dim3(256), 0, d->s, cnt,
+                       d->X[parity], d->dx, d->x, B, nullptr);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(d->s));
   }
@@ -971,9 +1274,9 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
   int code = 0, col = 0, inst = 0;
   if ((st = pvglu_status(pv->plan, &code, &col, &inst)))
     return st;
-  if (code)
+  if (code) // recorded in the last working batch
     return fail(code, "batched Newton: status %d at column/element %d, instance %d",
-                code, col, inst);
+                code, col, cm.caller_instance(inst));
   HIP_TRY(hipMemcpy(v_module, d->v, sizeof(double) * nm * B,
                     hipMemcpyDeviceToHost));
   if (stats) {
@@ -1311,6 +1614,14 @@ extern "C" int pvglu_pv_set_newton_form(pvglu_pv *pv, int form) {
   if (!pv || (form != PVGLU_NEWTON_CORRECTION && form != PVGLU_NEWTON_DIRECT))
     return fail(PVGLU_E_ARG, "set_newton_form: bad arguments");
   pv->form = form;
+  return PVGLU_OK;
+}
+
+extern "C" int pvglu_pv_set_skip_frozen(pvglu_pv *pv, int on) {
+  begin_call();
+  if (!pv)
+    return fail(PVGLU_E_ARG, "set_skip_frozen: null");
+  pv->skip_frozen = on ? 1 : 0;
   return PVGLU_OK;
 }
 

@@ -41,6 +41,7 @@ struct pvglu_pv {
   std::vector<double> lin_b;
 
   int form = PVGLU_NEWTON_CORRECTION; // Newton step form of the drivers
+  int skip_frozen = 1; // batched Newton: skip tiles of frozen instances
   pvglu::NewtonDev *nd = nullptr;
   pvglu::DropinDev *dd = nullptr;
   pvglu::CloudDev *cd = nullptr;
