@@ -397,6 +397,16 @@ int pvglu_pv_get_newton_form(const pvglu_pv *pv);
  * instance per solve). */
 int pvglu_pv_set_skip_frozen(pvglu_pv *pv, int on);
 
+/* Batched Newton, correction form: on = 1 (the default) forms the companion
+ * matrix's series-string entries inside the refactor (each from the two
+ * module conductances it sums, in the stamping order, so bit-identical to
+ * the stamped values) instead of stamping them into A and reading them
+ * back; only the other entries are stamped.  Applies where the refactor's
+ * LU levels run as level launches (HYBRID, LEVEL); 0 stamps every entry.
+ * No counterpart in the reference (host assembly of every entry,
+ * PV/002b_sistema_electrico_base_matriz.cpp:183-203). */
+int pvglu_pv_set_stamp_on_load(pvglu_pv *pv, int on);
+
 /* Host drop-in Newton solve of one plant, with the reference's call sequence
  * (PV/000_PVPP_simulator.cpp:144-235): symbolic setup once, then per
  * iteration host assembly, permutation, CSC value fill, GPU refactor, GPU

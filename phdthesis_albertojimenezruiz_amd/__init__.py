@@ -313,6 +313,7 @@ _PV_SIG = {
     "pvglu_pv_set_newton_form": (C.c_int, [_VP, C.c_int]),
     "pvglu_pv_get_newton_form": (C.c_int, [_VP]),
     "pvglu_pv_set_skip_frozen": (C.c_int, [_VP, C.c_int]),
+    "pvglu_pv_set_stamp_on_load": (C.c_int, [_VP, C.c_int]),
 }
 
 
@@ -458,6 +459,12 @@ class PV:
         instances have all finished (default on; off runs every instance on
         every iteration, with bit-identical results)."""
         _check(_pvlib().pvglu_pv_set_skip_frozen(self._h, 1 if on else 0))
+
+    def set_stamp_on_load(self, on=True):
+        """Batched Newton (correction form): form the series-string matrix
+        entries inside the refactor instead of stamping them (default on;
+        bit-identical results either way)."""
+        _check(_pvlib().pvglu_pv_set_stamp_on_load(self._h, 1 if on else 0))
 
     def v0(self, batch=None):
         v = np.full(self.nmod, 600.0 / self.M)

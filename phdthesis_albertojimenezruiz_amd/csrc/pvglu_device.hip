@@ -63,6 +63,9 @@ struct DevicePlan {
   // instance activity flags of the launch being issued (masked launches
   // only; null otherwise)
   const int *act = nullptr;
+  // stamping on load of the launch being issued (stamped launches only)
+  const int *stamp_seg = nullptr;
+  const double *stamp_geq = nullptr, *stamp_gvs = nullptr;
   std::vector<void *> allocs;
   // component groups (first component of each group, 1 / 2 / 4 groups) and
   // the side streams and events of the grouped schedules
@@ -82,7 +85,8 @@ static Pat pat_of(const DevicePlan *d) {
              d->lu_fast,   d->u_rec,    d->u_seg,  d->u_run,     d->u_fast,
              d->l_rec,     d->l_seg,    d->l_run,  d->l_fast,    d->ut_rec,
              d->ut_seg,    d->ut_run,   d->ut_fast, d->lt_rec,   d->lt_seg,
-             d->lt_run,    d->lt_fast,  d->drf,    d->status, d->act};
+             d->lt_run,    d->lt_fast,  d->drf,    d->status, d->act,
+             d->stamp_seg, d->stamp_geq, d->stamp_gvs};
 }
 static Lev lev_of(const DevLevels &L) {
   return Lev{L.level_chain_ptr, L.chain_col_ptr, L.cols,
@@ -528,8 +532,13 @@ static int launch_steps_w(pvglu_plan *P, int mode, bool hybrid, int G,
       if (!it.run) {
         const DevLevels &L = *sets[it.set];
         const int *r = &L.h_grp[gi][((size_t)g * L.nlev + it.lv) * 2];
-        launch_one_level<W>(P, L, r[0], r[1] - r[0], it.op, A, LU, b, x, B,
-                            st, stream);
+        // a stamped call forms the string segments' A values on load in its
+        // LU level launches (stamped_lu_levels lists them)
+        const int op = (d->stamp_seg && it.op == OP_REFACTOR_FUSED)
+                           ? (int)OP_REFACTOR_STAMPED
+                           : it.op;
+        launch_one_level<W>(P, L, r[0], r[1] - r[0], op, A, LU, b, x, B, st,
+                            stream);
       } else {
         int slots = 1;
         while (slots < it.widest)
@@ -601,16 +610,28 @@ static int launch_component(pvglu_plan *P, int W, int mode, int threads,
 // skipped (their factors and solution keep their earlier values).
 static int run_numeric(pvglu_plan *P, int mode, const double *A, double *LU,
                        const double *b, double *x, int B, void *stream,
-                       const int *act = nullptr, bool fit = false) {
+                       const int *act = nullptr, bool fit = false,
+                       const StampOnLoad *so = nullptr) {
   begin_call();
   int st = require_device(P);
   if (st)
     return st;
-  struct MaskScope { // the flags apply to this call's launches only
+  struct MaskScope { // the flags and stamping apply to this call only
     DevicePlan *d;
-    MaskScope(DevicePlan *d, const int *a) : d(d) { d->act = a; }
-    ~MaskScope() { d->act = nullptr; }
-  } scope(P->dev, act);
+    MaskScope(DevicePlan *d, const int *a, const StampOnLoad *so) : d(d) {
+      d->act = a;
+      if (so) {
+        d->stamp_seg = so->seg;
+        d->stamp_geq = so->geq;
+        d->stamp_gvs = so->gvs;
+      }
+    }
+    ~MaskScope() {
+      d->act = nullptr;
+      d->stamp_seg = nullptr;
+      d->stamp_geq = d->stamp_gvs = nullptr;
+    }
+  } scope(P->dev, act, so);
   if (B <= 0)
     return fail(PVGLU_E_ARG, "batch must be positive");
   if ((mode != 2 && mode != 3 && (!A || !LU)) ||
@@ -646,6 +667,30 @@ int refactor_solve_masked(pvglu_plan *P, const double *a, double *lu,
                           const double *b, double *x, int batch, void *stream,
                           const int *active) {
   return run_numeric(P, 1, a, lu, b, x, batch, stream, active, true);
+}
+
+int refactor_solve_stamped(pvglu_plan *P, const StampOnLoad &so,
+                           const double *a, double *lu, const double *b,
+                           double *x, int batch, void *stream,
+                           const int *active) {
+  if (!so.seg || !so.geq || !so.gvs)
+    return fail(PVGLU_E_ARG, "stamped refactor: null stamping arrays");
+  return run_numeric(P, 1, a, lu, b, x, batch, stream, active, true, &so);
+}
+
+int stamped_lu_levels(pvglu_plan *P, int batch, std::vector<int> &levels) {
+  levels.clear();
+  int st = require_device(P);
+  if (st)
+    return st;
+  int variant, W, threads;
+  pick_default_schedule(P, batch, variant, W, threads, true);
+  if (variant == PVGLU_SCHED_COMPONENT || !P->fuse_lsolve)
+    return PVGLU_OK;
+  for (const Item &it : build_items(P, 1, variant == PVGLU_SCHED_HYBRID))
+    if (!it.run && it.set == 0)
+      levels.push_back(it.lv);
+  return PVGLU_OK;
 }
 } // namespace pvglu
 

@@ -34,6 +34,27 @@ int refactor_solve_masked(pvglu_plan *P, const double *a, double *lu,
                           const double *b, double *x, int batch, void *stream,
                           const int *active);
 
+// Stamping on load: seg holds kStampRec ints per kernel position of the LU
+// levels (pvglu_launch.h), geq the module conductances [module][instance],
+// gvs the homotopy conductance per instance.
+struct StampOnLoad {
+  const int *seg;
+  const double *geq;
+  const double *gvs;
+};
+// refactor_solve_masked whose LU level launches (the levels that
+// stamped_lu_levels lists for this batch) form the A values of the string
+// segments with a stamping record from geq and gvs; every other A value is
+// read from `a` as usual.
+int refactor_solve_stamped(pvglu_plan *P, const StampOnLoad &so,
+                           const double *a, double *lu, const double *b,
+                           double *x, int batch, void *stream,
+                           const int *active);
+// The LU levels that a refactor + solve of `batch` instances runs as level
+// launches under the plan's schedule (empty for COMPONENT, or without the
+// fused forward substitution): the levels a stamped call stamps on load.
+int stamped_lu_levels(pvglu_plan *P, int batch, std::vector<int> &levels);
+
 // Levels of chains of columns in the flattened form of
 // PV/008_convert_chains_columns_levels_into_vector.cpp:4-30.
 struct LevelSet {

@@ -540,6 +540,80 @@ __device__ __forceinline__ void tri_lu_segment(
   p3 = 0.0;
 }
 
+// tri_lu_segment with the fused forward row, its A values stamped on load:
+// per column i the U-above entry is the conductance g_u of one module, the
+// L-below entry g_l of another and the diagonal their sum, each term signed
+// and, on homotopy stages (gv > 0), followed by the same sign times gv, in
+// the stamping map's order -- the operations of the stamping gather
+// (k_stamp_a) with its zero linear part, so the values are bit-identical to
+// the stamped matrix's.  One load per module conductance replaces the three
+// loads of A per column (and the stamping kernel's write of them).
+template <int D, int KC = -1, bool STREAM = false>
+__device__ __forceinline__ void tri_lu_segment_stamped(
+    const int (&sr)[kStampRec], int slot0, int col0, int len, int dslot,
+    int dcol, int k, const double *geq, double gv, double *LU,
+    const double *b, double *x, size_t B, int inst, double &p0, double &p1,
+    double &p2, double &p3, double &yprev, int &badcol) {
+  if (KC >= 0)
+    k = KC;
+  const int mu0 = sr[1], dmu = sr[2], ml0 = sr[3], dml = sr[4], sb = sr[5];
+  const double su = (sb & 1) ? -1.0 : 1.0, sl = (sb & 2) ? -1.0 : 1.0,
+               s1 = (sb & 4) ? -1.0 : 1.0, s2 = (sb & 8) ? -1.0 : 1.0;
+  const bool first_u = (sb & 16) != 0;
+  double pl = sel4(p0, p1, p2, p3, k);
+  double v0 = 0.0, v1 = 0.0, v2 = 0.0, minpiv = 1.0;
+  auto term = [&](double acc, double sg, double g) {
+    acc = acc + sg * g;
+    if (gv > 0)
+      acc = acc + sg * gv;
+    return acc;
+  };
+  pipeline<D, ColVals>(
+      len,
+      [&](ColVals &c, int i) {
+        i = min(i, len - 1);
+        // cached loads: a string's next column reads one of the two again
+        c.a0 = geq[(size_t)(mu0 + i * dmu) * B + inst];
+        c.a2 = geq[(size_t)(ml0 + i * dml) * B + inst];
+        c.bb = ldv<STREAM>(&b[(size_t)(col0 + i * dcol) * B + inst]);
+      },
+      [&](const ColVals &c, int i) {
+        const size_t s = (size_t)(slot0 + i * dslot) * B + inst;
+        const double zero = 0.0;
+        const double au = term(zero, su, c.a0);
+        const double al = term(zero, sl, c.a2);
+        const double ad = first_u ? term(term(zero, s1, c.a0), s2, c.a2)
+                                  : term(term(zero, s1, c.a2), s2, c.a0);
+        const double u = au;
+        const double piv = ad - pl * u;
+        minpiv = fmin(minpiv, fabs(piv));
+        const double lo = al / piv;
+        LU[s] = u;
+        LU[s + B] = piv;
+        stv<STREAM>(&LU[s + 2 * B], lo);
+        const double y = c.bb - pl * yprev;
+        stv<STREAM>(&x[(size_t)(col0 + i * dcol) * B + inst], y);
+        yprev = y;
+        v0 = u;
+        v1 = piv;
+        v2 = lo;
+        if constexpr (KC == 2)
+          pl = lo;
+        else
+          pl = k == 0 ? u : (k == 1 ? piv : (k == 2 ? lo : 0.0));
+      });
+  if (minpiv < 1e-8)
+    for (int i = 0; i < len; ++i)
+      if (fabs(LU[(size_t)(slot0 + i * dslot + 1) * B + inst]) < 1e-8) {
+        badcol = col0 + i * dcol;
+        break;
+      }
+  p0 = v0;
+  p1 = v1;
+  p2 = v2;
+  p3 = 0.0;
+}
+
 // Star column (column flag bit 1; the DC bus): only the diagonal has
 // updates, so the entries above the diagonal are their A values, every U
 // operand of the diagonal is such an A value, and the entries below the
@@ -685,13 +759,19 @@ __device__ __forceinline__ void pure_row_segment(const int (&f)[8], int len,
 
 // p0..p3 / yprev: the previous column's window (zero at a chain's start),
 // updated to this run's last column.
-template <int W, bool FUSE, int D = kLuDepth, bool STREAM = false>
+// SF: the launch stamps the A values of the string segments that have a
+// stamping record (OP_REFACTOR_STAMPED, FUSE).
+template <int W, bool FUSE, int D = kLuDepth, bool STREAM = false,
+          bool SF = false>
 __device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
                                               const double *A, double *LU,
                                               const double *b, double *x,
                                               size_t B, int inst, double &p0,
                                               double &p1, double &p2,
                                               double &p3, double &yprev) {
+  double gv = 0.0;
+  if constexpr (SF)
+    gv = P.gvs[inst];
   for (int t = t0; t < t1;) {
     const int *sg = P.lu_seg + (size_t)t * 4;
     const int len = uniform<W>(sg[0]);
@@ -705,6 +785,25 @@ __device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
     const int sflag = uniform<W>(sg[3]);
     if (sflag & (FUSE ? 8 : 4)) {
       int badcol = -1;
+      if constexpr (SF) {
+        int sr[kStampRec];
+        load_record<W>(P.stamp_seg + (size_t)t * kStampRec, sr);
+        if (sr[0]) {
+          if (((sflag >> 4) & 3) == 2)
+            tri_lu_segment_stamped<D, 2, STREAM>(sr, f[0], f[1], len, dslot,
+                                                 dcol, 2, P.geq, gv, LU, b, x,
+                                                 B, inst, p0, p1, p2, p3,
+                                                 yprev, badcol);
+          else
+            tri_lu_segment_stamped<D, -1, STREAM>(
+                sr, f[0], f[1], len, dslot, dcol, (sflag >> 4) & 3, P.geq, gv,
+                LU, b, x, B, inst, p0, p1, p2, p3, yprev, badcol);
+          if (badcol >= 0)
+            record_status(P.status, PVGLU_E_SINGULAR, badcol, inst);
+          t += len;
+          continue;
+        }
+      }
       if (((sflag >> 4) & 3) == 2)
         tri_lu_segment<FUSE, D, 2, STREAM>(f[0], f[1], len, dslot, dcol, 2, A,
                                            LU, b, x, B, inst, p0, p1, p2, p3,
@@ -884,12 +983,14 @@ __device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
   constexpr int DR = NAR ? kNarrowRowDepth : kRowDepth;
   const int t0 = uniform<W>(L.chain_col_ptr[chain]);
   const int t1 = uniform<W>(L.chain_col_ptr[chain + 1]);
-  if constexpr (OP == OP_REFACTOR_FUSED || OP == OP_REFACTOR) {
-    constexpr bool FUSE = OP == OP_REFACTOR_FUSED;
+  if constexpr (OP == OP_REFACTOR_FUSED || OP == OP_REFACTOR ||
+                OP == OP_REFACTOR_STAMPED) {
+    constexpr bool FUSE = OP != OP_REFACTOR;
+    constexpr bool SF = OP == OP_REFACTOR_STAMPED;
     double p0 = 0, p1 = 0, p2 = 0, p3 = 0, yprev = 0;
     if (uniform<W>(P.lu_fast[chain])) {
-      fast_lu_chain<W, FUSE, DL, STREAM>(P, t0, t1, A, LU, b, x, B, inst, p0,
-                                         p1, p2, p3, yprev);
+      fast_lu_chain<W, FUSE, DL, STREAM, SF>(P, t0, t1, A, LU, b, x, B, inst,
+                                             p0, p1, p2, p3, yprev);
       return;
     }
     // mixed chain: register runs between generic columns; a run's first
@@ -897,8 +998,8 @@ __device__ __forceinline__ void run_chain(const Pat &P, const Lev &L, int chain,
     for (int t = t0; t < t1;) {
       const int e = uniform<W>(P.lu_run[t]);
       if (e > t) {
-        fast_lu_chain<W, FUSE, DL, STREAM>(P, t, e, A, LU, b, x, B, inst, p0,
-                                           p1, p2, p3, yprev);
+        fast_lu_chain<W, FUSE, DL, STREAM, SF>(P, t, e, A, LU, b, x, B, inst,
+                                               p0, p1, p2, p3, yprev);
         t = e;
       } else {
         const int c = uniform<W>(L.cols[t]);
@@ -1217,6 +1318,7 @@ void Launch<W>::level(const Pat &pat, const Lev &lev, int op, int first,
   switch (op) {
   case OP_REFACTOR: PVGLU_LVL(OP_REFACTOR); break;
   case OP_REFACTOR_FUSED: PVGLU_LVL(OP_REFACTOR_FUSED); break;
+  case OP_REFACTOR_STAMPED: PVGLU_LVL(OP_REFACTOR_STAMPED); break;
   case OP_LSOLVE: PVGLU_LVL(OP_LSOLVE); break;
   case OP_USOLVE: PVGLU_LVL(OP_USOLVE); break;
   case OP_UT: PVGLU_LVL(OP_UT); break;

@@ -54,7 +54,23 @@ struct Pat {
   // work: the batched Newton's frozen instances, whose system and solution
   // no longer change.
   const int *act;
+  // Stamping on load (OP_REFACTOR_STAMPED launches only, null otherwise):
+  // per kernel position t of the LU levels, kStampRec ints describing how
+  // the A values of the series-string segment starting there follow from
+  // the module conductances geq [module][instance] and the homotopy
+  // conductance gvs[instance] (see tri_lu_segment_stamped).
+  const int *stamp_seg;
+  const double *geq;
+  const double *gvs;
 };
+
+// Ints per kernel position of a stamping record: [0] 1 when the segment's A
+// values are stamped on load; [1] module of the U-above entry of the first
+// column, [2] its step per column; [3] module of the L-below entry, [4] its
+// step; [5] bits: 0/1/2/3 sign of the U-above / L-below / first diagonal /
+// second diagonal term negative, 4 the diagonal's first term is the U-above
+// module.  The entries' linear parts are zero.
+constexpr int kStampRec = 8;
 
 struct Lev {
   const int *__restrict__ level_chain_ptr;
@@ -66,7 +82,18 @@ struct Lev {
   const int *__restrict__ comp_lv_level;
 };
 
-enum Op { OP_REFACTOR = 0, OP_REFACTOR_FUSED, OP_LSOLVE, OP_USOLVE, OP_UT, OP_LT };
+// OP_REFACTOR_STAMPED: OP_REFACTOR_FUSED whose string segments with a
+// stamping record form their A values from the module conductances instead
+// of loading them (Pat::stamp_seg).
+enum Op {
+  OP_REFACTOR = 0,
+  OP_REFACTOR_FUSED,
+  OP_LSOLVE,
+  OP_USOLVE,
+  OP_UT,
+  OP_LT,
+  OP_REFACTOR_STAMPED
+};
 
 // HYBRID schedule: a run of narrow levels (few chains) in one launch.  Block
 // -> (component, tile); the four level sets are walked in order

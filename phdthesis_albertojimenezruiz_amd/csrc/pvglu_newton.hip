@@ -88,6 +88,16 @@ struct NewtonDev {
   double tol = 1e-10;    // captured into the graph; re-captured when changed
   int form = PVGLU_NEWTON_CORRECTION; // form the graphs were captured for
   int skip = 1; // frozen-tile skipping the graphs were captured with
+  // stamping on load (correction form): the records of the LU string
+  // segments (kStampRec ints per kernel position; host copy kept for the
+  // coverage of each batch), the A slots the refactor does not stamp for the
+  // captured batch (k_stamp_a_list writes them) and whether the captured
+  // graphs use it
+  int *stamp_seg = nullptr, *a_part = nullptr;
+  std::vector<int> h_stamp_seg;
+  int npart = 0;
+  bool stamped = false; // the captured graphs stamp on load
+  int fused = 1;        // stamping on load requested when captured
   hipStream_t s = nullptr;
   // one graph per iteration; the correction form alternates two (parity p
   // reads node voltages X[p] and writes X[1 - p])
@@ -438,6 +448,33 @@ __global__ void __launch_bounds__(256)
                     (int)blockIdx.x * kStampRows, inst);
 }
 
+// k_stamp_a over the slots list[0..cnt) only (the slots a stamping-on-load
+// refactor does not form itself), same operations per slot.
+__global__ void __launch_bounds__(256)
+    k_stamp_a_list(int cnt, const int *__restrict__ list, int B,
+                   const double *__restrict__ base, const int *__restrict__ ptr,
+                   const int *__restrict__ mod, const int *__restrict__ sgn,
+                   const double *__restrict__ geq,
+                   const double *__restrict__ gvp, double *__restrict__ A,
+                   const int *active) {
+  const int inst = (int)(blockIdx.y * blockDim.x + threadIdx.x);
+  if (inst >= B || (active && !active[inst]))
+    return;
+  const double gv = gvp[inst];
+  const int i0 = (int)blockIdx.x * kStampRows;
+  for (int r = 0; r < kStampRows && i0 + r < cnt; ++r) {
+    const int q = list[i0 + r];
+    double acc = base[q];
+    for (int k = ptr[q]; k < ptr[q + 1]; ++k) {
+      const double sg = (double)sgn[k];
+      acc = acc + sg * geq[(size_t)mod[k] * B + inst];
+      if (gv > 0)
+        acc = acc + sg * gv;
+    }
+    A[(size_t)q * B + inst] = acc;
+  }
+}
+
 // b[r] = linear sources - sum over modules of sign * I_eq
 __global__ void __launch_bounds__(256)
     k_stamp_b(int n, int B, const double *__restrict__ base,
@@ -712,6 +749,56 @@ int newton_setup(pvglu_pv *pv) {
   std::vector<int> a_ptr, a_mod, a_sgn, b_ptr, b_mod, b_sgn;
   flat(as, a_ptr, a_mod, a_sgn);
   flat(bs, b_ptr, b_mod, b_sgn);
+  // Stamping-on-load records: a tridiagonal string segment of the refactor
+  // (with the fused forward row) whose three entries per column are, in
+  // every column, one module's conductance (U above), another's (L below)
+  // and their sum (diagonal), with zero linear parts, the modules advancing
+  // by constant steps and the signs and the diagonal's term order fixed.
+  std::vector<int> sseg((size_t)n * kStampRec, 0);
+  for (int t = 0; t < n; ++t) {
+    const int *sg = &P->fast_lu_seg[(size_t)t * 4];
+    const int len = sg[0];
+    if (len <= 0 || !(sg[3] & 8))
+      continue;
+    const int slot0 = P->fast_lu_rec[(size_t)t * 16], dslot = sg[1];
+    int r[kStampRec] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool ok = true;
+    for (int i = 0; i < len && ok; ++i) {
+      const int q = slot0 + i * dslot;
+      auto cnt = [&](int qq) { return a_ptr[qq + 1] - a_ptr[qq]; };
+      if (base_a[q] != 0.0 || base_a[q + 1] != 0.0 || base_a[q + 2] != 0.0 ||
+          cnt(q) != 1 || cnt(q + 1) != 2 || cnt(q + 2) != 1) {
+        ok = false;
+        break;
+      }
+      const int mu = a_mod[a_ptr[q]], ml = a_mod[a_ptr[q + 2]];
+      const int m1 = a_mod[a_ptr[q + 1]], m2 = a_mod[a_ptr[q + 1] + 1];
+      const bool first_u = m1 == mu && m2 == ml;
+      if (!first_u && !(m1 == ml && m2 == mu)) {
+        ok = false;
+        break;
+      }
+      const int bits = (a_sgn[a_ptr[q]] < 0 ? 1 : 0) |
+                       (a_sgn[a_ptr[q + 2]] < 0 ? 2 : 0) |
+                       (a_sgn[a_ptr[q + 1]] < 0 ? 4 : 0) |
+                       (a_sgn[a_ptr[q + 1] + 1] < 0 ? 8 : 0) |
+                       (first_u ? 16 : 0);
+      if (i == 0) {
+        r[1] = mu;
+        r[3] = ml;
+        r[5] = bits;
+      } else if (i == 1) {
+        r[2] = mu - r[1];
+        r[4] = ml - r[3];
+      }
+      ok = ok && bits == r[5] && mu == r[1] + i * r[2] &&
+           ml == r[3] + i * r[4];
+    }
+    if (ok) {
+      r[0] = 1;
+      std::copy(r, r + kStampRec, &sseg[(size_t)t * kStampRec]);
+    }
+  }
   std::vector<int> pos(nm), neg(nm), rpos(nm), rneg(nm);
   for (int e = 0; e < nm; e++) {
     pos[e] = pv->mod_pos[e] >= 0 ? colmap[pv->mod_pos[e]] : -1;
@@ -771,8 +858,11 @@ int newton_setup(pvglu_pv *pv) {
       (st = dupload(A, &d->l_g, l_g)) || (st = dupload(A, &d->l_e, l_e)) ||
       (st = dupload(A, &d->ladder,
                     std::vector<double>(kLadder, kLadder + 12))) ||
-      (st = dupload(A, &d->lk, std::vector<double>(pv->nmod, 0.0))))
+      (st = dupload(A, &d->lk, std::vector<double>(pv->nmod, 0.0))) ||
+      (st = dupload(A, &d->stamp_seg, sseg)) ||
+      (st = dupload(A, &d->a_part, std::vector<int>(std::max(nnz, 1), 0))))
     return st;
+  d->h_stamp_seg.swap(sseg);
   HIP_TRY(hipStreamCreate(&d->s));
   if (pv->nmod > 0) {
     hipLaunchKernelGGL(k_module_lk, dim3((pv->nmod + 255) / 256), dim3(256), 0,
@@ -869,6 +959,49 @@ inline double noise_of(int form) {
   return form == PVGLU_NEWTON_DIRECT ? kNoiseDirect : kNoiseCorrection;
 }
 
+// Stamping on load for a batch of B (correction form, switch on): the LU
+// levels that run as level launches stamp their string segments with a
+// record; the other A slots are listed for k_stamp_a_list.  Called before
+// capture (it uploads the list).
+int prepare_stamping(pvglu_pv *pv, int B) {
+  NewtonDev *d = pv->nd;
+  d->stamped = false;
+  d->npart = 0;
+  if (!pv->fused_stamping || d->form != PVGLU_NEWTON_CORRECTION)
+    return PVGLU_OK;
+  pvglu_plan *P = pv->plan;
+  std::vector<int> levels;
+  int st = stamped_lu_levels(P, B, levels);
+  if (st)
+    return st;
+  if (levels.empty())
+    return PVGLU_OK;
+  std::vector<char> covered(P->nnz, 0);
+  const KernelLevels &K = P->k_lu;
+  for (int lv : levels)
+    for (int ch = K.level_chain_ptr[lv]; ch < K.level_chain_ptr[lv + 1]; ++ch)
+      for (int t = K.chain_col_ptr[ch]; t < K.chain_col_ptr[ch + 1]; ++t) {
+        if (!d->h_stamp_seg[(size_t)t * kStampRec])
+          continue;
+        const int len = P->fast_lu_seg[(size_t)t * 4];
+        const int dslot = P->fast_lu_seg[(size_t)t * 4 + 1];
+        const int slot0 = P->fast_lu_rec[(size_t)t * 16];
+        for (int i = 0; i < len; ++i)
+          for (int k = 0; k < 3; ++k)
+            covered[slot0 + i * dslot + k] = 1;
+      }
+  std::vector<int> part;
+  for (int q = 0; q < P->nnz; ++q)
+    if (!covered[q])
+      part.push_back(q);
+  if (!part.empty())
+    HIP_TRY(hipMemcpy(d->a_part, part.data(), sizeof(int) * part.size(),
+                      hipMemcpyHostToDevice));
+  d->npart = (int)part.size();
+  d->stamped = true;
+  return PVGLU_OK;
+}
+
 // One Newton iteration on d->s.  Direct form: stamping of A and b, solve for
 // the node voltages x.  Correction form (parity p): stamping of A, the
 // branch-current residual at X[p] + dx (writing those node voltages to
@@ -891,10 +1024,16 @@ int launch_iteration(pvglu_pv *pv, int B, int parity) {
                                     d->skip ? d->active : nullptr)))
       return st;
   } else {
-    const RowGrid ga = stamp_grid(nnz, B), gb = stamp_grid(n, B);
-    hipLaunchKernelGGL(k_stamp_a, ga.grid, ga.block, 0, d->s, nnz, B,
-                       d->base_a, d->a_ptr, d->a_mod, d->a_sgn, d->geq, d->gvs,
-                       d->A, d->active);
+    const RowGrid ga = stamp_grid(d->stamped ? d->npart : nnz, B),
+                  gb = stamp_grid(n, B);
+    if (!d->stamped)
+      hipLaunchKernelGGL(k_stamp_a, ga.grid, ga.block, 0, d->s, nnz, B,
+                         d->base_a, d->a_ptr, d->a_mod, d->a_sgn, d->geq,
+                         d->gvs, d->A, d->active);
+    else if (d->npart > 0) // the rest is stamped on load by the refactor
+      hipLaunchKernelGGL(k_stamp_a_list, ga.grid, ga.block, 0, d->s, d->npart,
+                         d->a_part, B, d->base_a, d->a_ptr, d->a_mod,
+                         d->a_sgn, d->geq, d->gvs, d->A, d->active);
     // x of this iteration = the previous one plus its correction (every
     // instance: a frozen one's correction is zero after its first replay)
     const long cnt = (long)n * B;
@@ -907,9 +1046,16 @@ int launch_iteration(pvglu_pv *pv, int B, int parity) {
                        d->b, d->active);
     HIP_TRY(hipGetLastError());
     // a tile of frozen instances is skipped: k_add cleared their corrections
-    if ((st = refactor_solve_masked(pv->plan, d->A, d->LU, d->b, d->dx, B,
-                                    d->s, d->skip ? d->active : nullptr)))
+    const int *act = d->skip ? d->active : nullptr;
+    if (d->stamped) {
+      const StampOnLoad so{d->stamp_seg, d->geq, d->gvs};
+      if ((st = refactor_solve_stamped(pv->plan, so, d->A, d->LU, d->b, d->dx,
+                                       B, d->s, act)))
+        return st;
+    } else if ((st = refactor_solve_masked(pv->plan, d->A, d->LU, d->b, d->dx,
+                                           B, d->s, act))) {
       return st;
+    }
     xnew = d->X[1 - parity];
     dx = d->dx;
   }
@@ -958,7 +1104,10 @@ int build_graph(pvglu_pv *pv, int B) {
   NewtonDev *d = pv->nd;
   if (d->exec)
     return PVGLU_OK;
-  int st = capture_one(pv, B, 0, &d->graph, &d->exec);
+  int st = prepare_stamping(pv, B);
+  if (st)
+    return st;
+  st = capture_one(pv, B, 0, &d->graph, &d->exec);
   if (!st && d->form == PVGLU_NEWTON_CORRECTION)
     st = capture_one(pv, B, 1, &d->graph2, &d->exec2);
   if (st)
@@ -1199,12 +1348,13 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
     return st;
   const int nst = ladder ? 12 : 1;
   if (d->exec && (d->nstages != nst || d->tol != tol || d->form != pv->form ||
-                  d->skip != pv->skip_frozen))
+                  d->skip != pv->skip_frozen || d->fused != pv->fused_stamping))
     drop_graphs(d);
   d->nstages = nst;
   d->tol = tol;
   d->form = pv->form;
   d->skip = pv->skip_frozen;
+  d->fused = pv->fused_stamping;
   const bool corr = d->form == PVGLU_NEWTON_CORRECTION;
   hipLaunchKernelGGL(k_newton_init, grid_for(B), dim3(256), 0, d->s, B,
                      kLadder[12 - nst], d->stage, d->gvs, d->active, d->stall,
@@ -1622,6 +1772,14 @@ extern "C" int pvglu_pv_set_skip_frozen(pvglu_pv *pv, int on) {
   if (!pv)
     return fail(PVGLU_E_ARG, "set_skip_frozen: null");
   pv->skip_frozen = on ? 1 : 0;
+  return PVGLU_OK;
+}
+
+extern "C" int pvglu_pv_set_stamp_on_load(pvglu_pv *pv, int on) {
+  begin_call();
+  if (!pv)
+    return fail(PVGLU_E_ARG, "set_stamp_on_load: null");
+  pv->fused_stamping = on ? 1 : 0;
   return PVGLU_OK;
 }
 

@@ -42,6 +42,7 @@ struct pvglu_pv {
 
   int form = PVGLU_NEWTON_CORRECTION; // Newton step form of the drivers
   int skip_frozen = 1; // batched Newton: skip tiles of frozen instances
+  int fused_stamping = 1; // batched Newton: string A values stamped on load
   pvglu::NewtonDev *nd = nullptr;
   pvglu::DropinDev *dd = nullptr;
   pvglu::CloudDev *cd = nullptr;

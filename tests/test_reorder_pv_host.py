@@ -118,3 +118,28 @@ def test_module_current_matches_oracle_implicit_solve():
         assert I == pytest.approx(oI, rel=1e-12, abs=1e-12)
         assert dV == pytest.approx(odV, rel=1e-10, abs=1e-14)
         assert dI == pytest.approx(odI, rel=1e-10, abs=1e-14)
+
+
+def test_module_current_against_mpmath_lambert_w():
+    """The library's explicit module current (single-precision start, one
+    double-precision Fritsch-Shafer-Crowley step) against the same formula
+    evaluated with a 30-digit Lambert W, over ln(theta) from about -60 (the
+    small-argument series) to about 2600."""
+    mp = pytest.importorskip("mpmath")
+    mp.mp.dps = 30
+    Rs, I0, a = mp.mpf("0.3"), mp.mpf("1e-10"), mp.mpf("1.5")
+    rng = np.random.default_rng(5)
+    Vs = np.concatenate([rng.uniform(-60.0, 60.0, 300), rng.uniform(60.0, 4000.0, 60)])
+    for V in Vs:
+        Iph = rng.uniform(0.0, 9.0)
+        rsh = float(np.exp(rng.uniform(np.log(200), np.log(600))))
+        I, dV, _ = pv.module_current(V, Iph, rsh)
+        R, Vm, Ip = mp.mpf(rsh), mp.mpf(float(V)), mp.mpf(float(Iph))
+        lnth = mp.log(Rs * R * I0 / (a * (Rs + R))) + R * (Rs * (Ip + I0) + Vm) / (a * (Rs + R))
+        W = mp.lambertw(mp.e ** lnth).real
+        Ie = (R * (Ip + I0) - Vm) / (Rs + R) - a / Rs * W
+        dVe = -1 / (Rs + R) - R / (Rs * (Rs + R)) * W / (1 + W)
+        # the two terms of I, each rounded at its own magnitude
+        scale = 1.0 + abs(float((R * (Ip + I0) - Vm) / (Rs + R))) + abs(float(a / Rs * W))
+        assert abs(I - float(Ie)) <= 2e-14 * scale, (V, Iph, rsh)
+        assert abs(dV - float(dVe)) <= 1e-14 * (1.0 + abs(float(dVe))), (V, Iph, rsh)
