@@ -439,21 +439,47 @@ __device__ __forceinline__ void pure_lu_col(const ColVals &c, int slot0,
 // are issued before step i computes, into a ring of D + 1 register buffers
 // indexed by unrolled constants (no copies of registers with loads in
 // flight); LD clamps its step to the segment.
+//
+// The steps run in whole rounds of D + 1 (one exit per round, none inside),
+// the first round peeled, then at most D steps whose loads were issued by
+// the last round.  The compiler derives each step's memory-counter wait
+// from the operations issued since its loads; at a loop header it merges
+// the entry and back-edge states, so (a) an exit after every step
+// (register copies merged at the back edge) and (b) a loop entered from the
+// D-load prologue (fewer operations in flight than a round leaves) each made
+// it wait for nearly every prefetch once per round (vmcnt(4) and vmcnt(35)
+// at the head of the tridiagonal loop at D = 8); with the peeled round the
+// loop is entered in its steady state and every step waits only for its own
+// loads (vmcnt(62)).  The scheduling barriers keep a step's prefetch ahead of
+// its arithmetic: without them the scheduler moved a round's arithmetic
+// above the next round's loads, draining the pipeline once per round.
 template <int D, class T, class LD, class RUN>
 __device__ __forceinline__ void pipeline(int len, LD ld, RUN run) {
   T buf[D + 1];
 #pragma unroll
   for (int k = 0; k < D; ++k)
     ld(buf[k], k);
-  for (int i = 0;;) {
+  auto round = [&](int i) {
 #pragma unroll
     for (int u = 0; u <= D; ++u) {
-      ld(buf[(u + D) % (D + 1)], i + D);
-      run(buf[u], i);
-      if (++i >= len)
-        return;
+      ld(buf[(u + D) % (D + 1)], i + u + D);
+      __builtin_amdgcn_sched_barrier(0);
+      run(buf[u], i + u);
+      __builtin_amdgcn_sched_barrier(0);
     }
+  };
+  int i = 0;
+  if (len > D) {
+    round(0);
+    for (i = D + 1; i + D < len; i += D + 1)
+      round(i);
   }
+  // buf[u] holds step i + u (loaded by the last round, or by the prologue)
+  const int r = len - i; // 0 .. D
+#pragma unroll
+  for (int u = 0; u < D; ++u)
+    if (u < r)
+      run(buf[u], i + u);
 }
 
 // Prefetch depth of the refactor and substitution register pipelines: 2 in
