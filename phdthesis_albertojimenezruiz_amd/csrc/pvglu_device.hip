@@ -65,7 +65,9 @@ struct DevicePlan {
   const int *act = nullptr;
   // stamping on load of the launch being issued (stamped launches only)
   const int *stamp_seg = nullptr;
-  const double *stamp_geq = nullptr, *stamp_gvs = nullptr;
+  const double *stamp_geq = nullptr, *stamp_gvs = nullptr,
+               *stamp_ieq = nullptr, *stamp_xk = nullptr;
+  const int *stamp_live = nullptr;
   std::vector<void *> allocs;
   // component groups (first component of each group, 1 / 2 / 4 groups) and
   // the side streams and events of the grouped schedules
@@ -86,7 +88,8 @@ static Pat pat_of(const DevicePlan *d) {
              d->l_rec,     d->l_seg,    d->l_run,  d->l_fast,    d->ut_rec,
              d->ut_seg,    d->ut_run,   d->ut_fast, d->lt_rec,   d->lt_seg,
              d->lt_run,    d->lt_fast,  d->drf,    d->status, d->act,
-             d->stamp_seg, d->stamp_geq, d->stamp_gvs};
+             d->stamp_seg, d->stamp_geq, d->stamp_gvs, d->stamp_ieq,
+             d->stamp_xk,  d->stamp_live};
 }
 static Lev lev_of(const DevLevels &L) {
   return Lev{L.level_chain_ptr, L.chain_col_ptr, L.cols,
@@ -624,12 +627,16 @@ static int run_numeric(pvglu_plan *P, int mode, const double *A, double *LU,
         d->stamp_seg = so->seg;
         d->stamp_geq = so->geq;
         d->stamp_gvs = so->gvs;
+        d->stamp_ieq = so->ieq;
+        d->stamp_xk = so->xk;
+        d->stamp_live = so->live;
       }
     }
     ~MaskScope() {
       d->act = nullptr;
       d->stamp_seg = nullptr;
-      d->stamp_geq = d->stamp_gvs = nullptr;
+      d->stamp_geq = d->stamp_gvs = d->stamp_ieq = d->stamp_xk = nullptr;
+      d->stamp_live = nullptr;
     }
   } scope(P->dev, act, so);
   if (B <= 0)

@@ -36,11 +36,16 @@ int refactor_solve_masked(pvglu_plan *P, const double *a, double *lu,
 
 // Stamping on load: seg holds kStampRec ints per kernel position of the LU
 // levels (pvglu_launch.h), geq the module conductances [module][instance],
-// gvs the homotopy conductance per instance.
+// gvs the homotopy conductance per instance; ieq and live (optional) the
+// module branch currents and instance flags of the records that also form
+// the correction-form right-hand side of their rows.
 struct StampOnLoad {
   const int *seg;
   const double *geq;
   const double *gvs;
+  const double *ieq = nullptr; // branch currents (right-hand side on load)
+  const double *xk = nullptr;  // node voltages of the iteration
+  const int *live = nullptr;   // instance activity flags
 };
 // refactor_solve_masked whose LU level launches (the levels that
 // stamped_lu_levels lists for this batch) form the A values of the string

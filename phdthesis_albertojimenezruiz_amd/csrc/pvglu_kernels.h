@@ -319,6 +319,7 @@ __device__ __forceinline__ void load_record(const int *r, int (&f)[N]) {
 // A and b values of one refactor column.
 struct ColVals {
   double a0, a1, a2, a3, bb;
+  double xn; // stamped right-hand side on homotopy stages: next node voltage
 };
 
 template <bool FUSE>
@@ -487,7 +488,7 @@ __device__ __forceinline__ void pipeline(int len, LD ld, RUN run) {
 // narrow-run kernel, where few wavefronts run and each step of a chain would
 // otherwise wait for a memory round trip issued only two steps earlier.
 constexpr int kLuDepth = 2, kRowDepth = 2;
-constexpr int kNarrowLuDepth = 8, kNarrowRowDepth = 8;
+constexpr int kNarrowLuDepth = 8, kNarrowRowDepth = 12;
 
 template <int NE, bool FUSE, int D = kLuDepth, bool STREAM = false>
 __device__ __forceinline__ void pure_lu_segment(
@@ -574,18 +575,47 @@ __device__ __forceinline__ void tri_lu_segment(
 // (k_stamp_a) with its zero linear part, so the values are bit-identical to
 // the stamped matrix's.  One load per module conductance replaces the three
 // loads of A per column (and the stamping kernel's write of them).
-template <int D, int KC = -1, bool STREAM = false>
+//
+// RES (record sr[6] bit 0): the correction-form right-hand side of each row
+// is formed here too, from the branch currents i of the row's two modules
+// -- the residual kernel's operations: i + gv (x_pos - x_neg) on a homotopy
+// stage (gv > 0), then 0 - s0 i0 - s1 i1 in the row's branch order; zero for
+// a frozen instance -- instead of being read from b, which the residual
+// kernel then does not compute for these rows.  The modules' other
+// terminals are the segment's previous and next nodes (checked at plan
+// time); XH: some lane of the wavefront is on a homotopy stage, so the node
+// voltages xk of the previous, this and the next node are carried along.
+// A RES segment is a chained string (checked at plan time: column i's
+// L-below module is column i+1's U-above one), so each column loads the
+// conductance and branch current of one module, its L-below one, and takes
+// the U-above module's from the previous column's registers.
+template <int D, int KC = -1, bool STREAM = false, bool RES = false,
+          bool XH = false>
 __device__ __forceinline__ void tri_lu_segment_stamped(
     const int (&sr)[kStampRec], int slot0, int col0, int len, int dslot,
-    int dcol, int k, const double *geq, double gv, double *LU,
-    const double *b, double *x, size_t B, int inst, double &p0, double &p1,
-    double &p2, double &p3, double &yprev, int &badcol) {
+    int dcol, int k, const double *geq, const double *ieq, const double *xk,
+    bool live, double gv, double *LU, const double *b, double *x, size_t B,
+    int inst, double &p0, double &p1, double &p2, double &p3, double &yprev,
+    int &badcol) {
   if (KC >= 0)
     k = KC;
-  const int mu0 = sr[1], dmu = sr[2], ml0 = sr[3], dml = sr[4], sb = sr[5];
+  const int mu0 = sr[1], dmu = sr[2], ml0 = sr[3], dml = sr[4], sb = sr[5],
+            rb = sr[6];
   const double su = (sb & 1) ? -1.0 : 1.0, sl = (sb & 2) ? -1.0 : 1.0,
                s1 = (sb & 4) ? -1.0 : 1.0, s2 = (sb & 8) ? -1.0 : 1.0;
   const bool first_u = (sb & 16) != 0;
+  const bool rfirst_u = (rb & 2) != 0;
+  const double r0s = (rb & 4) ? -1.0 : 1.0, r1s = (rb & 8) ? -1.0 : 1.0;
+  double xprv = 0.0, xcur = 0.0; // node voltages of the previous / this row
+  if constexpr (RES && XH) {
+    xprv = xk[(size_t)(col0 - dcol) * B + inst];
+    xcur = xk[(size_t)col0 * B + inst];
+  }
+  double gup = 0.0, iup = 0.0; // chained: the U-above module's g and i
+  if constexpr (RES) {
+    gup = geq[(size_t)mu0 * B + inst];
+    iup = ieq[(size_t)mu0 * B + inst];
+  }
   double pl = sel4(p0, p1, p2, p3, k);
   double v0 = 0.0, v1 = 0.0, v2 = 0.0, minpiv = 1.0;
   auto term = [&](double acc, double sg, double g) {
@@ -599,17 +629,26 @@ __device__ __forceinline__ void tri_lu_segment_stamped(
       [&](ColVals &c, int i) {
         i = min(i, len - 1);
         // cached loads: a string's next column reads one of the two again
-        c.a0 = geq[(size_t)(mu0 + i * dmu) * B + inst];
+        if constexpr (!RES)
+          c.a0 = geq[(size_t)(mu0 + i * dmu) * B + inst];
         c.a2 = geq[(size_t)(ml0 + i * dml) * B + inst];
-        c.bb = ldv<STREAM>(&b[(size_t)(col0 + i * dcol) * B + inst]);
+        if constexpr (RES) { // the L-below module's branch current
+          c.a3 = ieq[(size_t)(ml0 + i * dml) * B + inst];
+          if constexpr (XH) // the next node (of the last column: sr[7])
+            c.xn = xk[(size_t)(i + 1 < len ? col0 + (i + 1) * dcol : sr[7]) *
+                          B + inst];
+        } else {
+          c.bb = ldv<STREAM>(&b[(size_t)(col0 + i * dcol) * B + inst]);
+        }
       },
       [&](const ColVals &c, int i) {
         const size_t s = (size_t)(slot0 + i * dslot) * B + inst;
         const double zero = 0.0;
-        const double au = term(zero, su, c.a0);
+        const double gu = RES ? gup : c.a0;
+        const double au = term(zero, su, gu);
         const double al = term(zero, sl, c.a2);
-        const double ad = first_u ? term(term(zero, s1, c.a0), s2, c.a2)
-                                  : term(term(zero, s1, c.a2), s2, c.a0);
+        const double ad = first_u ? term(term(zero, s1, gu), s2, c.a2)
+                                  : term(term(zero, s1, c.a2), s2, gu);
         const double u = au;
         const double piv = ad - pl * u;
         minpiv = fmin(minpiv, fabs(piv));
@@ -617,7 +656,31 @@ __device__ __forceinline__ void tri_lu_segment_stamped(
         LU[s] = u;
         LU[s + B] = piv;
         stv<STREAM>(&LU[s + 2 * B], lo);
-        const double y = c.bb - pl * yprev;
+        double rhs;
+        if constexpr (RES) {
+          double ia = rfirst_u ? iup : c.a3, ib = rfirst_u ? c.a3 : iup;
+          if constexpr (XH) {
+            // x_pos - x_neg of each module: this node and the other terminal
+            // (the previous node for the U-above module, the next for the
+            // L-below one); s > 0: this node is the module's positive one
+            const double xa = rfirst_u ? xprv : c.xn, xb = rfirst_u ? c.xn : xprv;
+            if (gv > 0) {
+              ia = ia + gv * (r0s > 0 ? xcur - xa : xa - xcur);
+              ib = ib + gv * (r1s > 0 ? xcur - xb : xb - xcur);
+            }
+            xprv = xcur;
+            xcur = c.xn;
+          }
+          double acc = 0.0;
+          acc = acc - r0s * ia;
+          acc = acc - r1s * ib;
+          rhs = live ? acc : 0.0;
+          gup = c.a2; // the next column's U-above module
+          iup = c.a3;
+        } else {
+          rhs = c.bb;
+        }
+        const double y = rhs - pl * yprev;
         stv<STREAM>(&x[(size_t)(col0 + i * dcol) * B + inst], y);
         yprev = y;
         v0 = u;
@@ -645,64 +708,63 @@ __device__ __forceinline__ void tri_lu_segment_stamped(
 // operand of the diagonal is such an A value, and the entries below the
 // diagonal are A / pivot; the operands' slots and rows are arithmetic
 // progressions, addressed from their first two values (no per-operand index
-// loads).  All operands of the diagonal's sum (and of the fused forward row)
-// are loaded together, before anything of the column is stored, and the sums
-// run branch-free over a fixed register budget of kStarMax updates: the
-// column costs one memory round trip instead of one per operand chunk.  Used
-// by the narrow-run kernel, where the column is on a latency-bound chain.
-// Operation order per entry = refactor_column's.
+// loads).  Used by the narrow-run kernel, where the column is on a
+// latency-bound chain.  Operation order per entry = refactor_column's.
 constexpr int kStarMax = kStarMaxUpdates;
+constexpr int kStarBelow = 4; // entries below the diagonal held in registers
 
 // The star column's 16-int record (built at plan time, at the column's
 // position in kernel order): q0, q1, diagonal slot, column flags, forward row
 // range r0 r1, number of updates, then (first, step) of the L slots, the U
 // slots and the forward rows.  One scalar load burst replaces the chain of
 // dependent metadata loads (column pointers -> update ranges -> operands).
-template <int W, bool FUSE>
-__device__ __forceinline__ void star_column(const Pat &P, int t, int c,
-                                            const double *A, double *LU,
-                                            const double *b, double *x,
-                                            size_t B, int inst, double &p0,
-                                            double &p1, double &p2,
-                                            double &p3, double &yprev) {
-  int f[16];
-  load_record<W>(P.lu_rec + (size_t)t * 16, f);
+//
+// Every load of the column is issued before anything is computed or stored:
+// the operands of the diagonal's sum (and of the fused forward row), the
+// diagonal, the right-hand side, up to kStarBelow entries below the diagonal
+// and the last four entries above it (the register window of the run that
+// follows).  When the U operands are the entries above the diagonal in slot
+// order (the plants' bus columns: u0 = q0, step 1, one update per entry),
+// those entries are stored from the operand registers.  The column then
+// costs one or two memory round trips (over 63 loads in flight the issue
+// waits) instead of one per 8-entry chunk of the copy and one for the
+// entries below (measured on the 40x24x28 plant's bus column: 9.3 us).  The
+// sums are unrolled to NS >= the update count (8, 16, 24 or 32), so few
+// clamped repeat loads fill the counter.
+template <int NS, int W, bool FUSE>
+__device__ __forceinline__ void star_body(const Pat &P, const int (&f)[16],
+                                          int c, const double *A, double *LU,
+                                          const double *b, double *x, size_t B,
+                                          int inst, double &p0, double &p1,
+                                          double &p2, double &p3,
+                                          double &yprev) {
   const int q0 = f[0], q1 = f[1], qd = f[2];
   const bool row_with_diag = FUSE && (f[3] & 1) != 0;
-  const int nd = f[6]; // 1 .. kStarMax
+  const int nd = f[6]; // 1 .. NS
   const int l0 = f[7], dl = f[8], u0 = f[9], du = f[10], y0 = f[11],
             dy = f[12];
-  double l[kStarMax], u[kStarMax], yv[kStarMax];
+  const int na = qd - q0, nb = q1 - qd - 1; // entries above / below
+  const bool above_is_u = u0 == q0 && du == 1 && nd == na;
+  double l[NS], u[NS], yv[NS], bl[kStarBelow], wa[4];
 #pragma unroll
-  for (int j = 0; j < kStarMax; ++j) {
+  for (int j = 0; j < NS; ++j) {
     const int k = min(j, nd - 1);
-    l[j] = LU[(size_t)(l0 + k * dl) * B + inst];
     u[j] = A[(size_t)(u0 + k * du) * B + inst];
+    l[j] = LU[(size_t)(l0 + k * dl) * B + inst];
     if (row_with_diag)
       yv[j] = x[(size_t)(y0 + k * dy) * B + inst];
   }
   double v = A[(size_t)qd * B + inst];
   double y = row_with_diag ? b[(size_t)c * B + inst] : 0.0;
-  // entries above the diagonal: no updates (LU = A), copied in chunks; the
-  // last four entries of the column are kept for the window
-  double w[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int qb = q0; qb < qd; qb += 8) {
-    double tv[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      tv[k] = A[(size_t)min(qb + k, qd - 1) * B + inst];
+  for (int k = 0; k < kStarBelow; ++k)
+    bl[k] = A[(size_t)min(qd + 1 + k, q1 - 1) * B + inst];
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
-      if (qb + k < qd) {
-        LU[(size_t)(qb + k) * B + inst] = tv[k];
-        w[0] = w[1];
-        w[1] = w[2];
-        w[2] = w[3];
-        w[3] = tv[k];
-      }
-  }
+  for (int k = 0; k < 4; ++k) // entry na - 4 + k above the diagonal, or 0
+    wa[k] = na - 4 + k >= 0 ? A[(size_t)(q0 + max(na - 4 + k, 0)) * B + inst]
+                            : 0.0;
 #pragma unroll
-  for (int j = 0; j < kStarMax; ++j) {
+  for (int j = 0; j < NS; ++j) {
     const double pv = v - l[j] * u[j];
     v = j < nd ? pv : v;
     if (row_with_diag) {
@@ -712,27 +774,62 @@ __device__ __forceinline__ void star_column(const Pat &P, int t, int c,
   }
   if (fabs(v) < 1e-8)
     record_status(P.status, PVGLU_E_SINGULAR, c, inst);
+  // entries above the diagonal: no updates (LU = A)
+  if (above_is_u) {
+#pragma unroll
+    for (int j = 0; j < NS; ++j)
+      if (j < nd)
+        LU[(size_t)(q0 + j) * B + inst] = u[j];
+  } else {
+    for (int qb = q0; qb < qd; qb += 8) {
+      double tv[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        tv[k] = A[(size_t)min(qb + k, qd - 1) * B + inst];
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (qb + k < qd)
+          LU[(size_t)(qb + k) * B + inst] = tv[k];
+    }
+  }
   LU[(size_t)qd * B + inst] = v;
-  w[0] = w[1];
-  w[1] = w[2];
-  w[2] = w[3];
-  w[3] = v;
   // entries below the diagonal: no updates, divided by the pivot
-  for (int qb = qd + 1; qb < q1; qb += 4) {
-    double tv[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      tv[k] = A[(size_t)min(qb + k, q1 - 1) * B + inst];
+  for (int k = 0; k < kStarBelow; ++k) {
+    bl[k] = bl[k] / v;
+    if (k < nb)
+      LU[(size_t)(qd + 1 + k) * B + inst] = bl[k];
+  }
+  // the window: the column's last four entries in slot order
+  double w[4];
+  if (nb <= kStarBelow) {
+    // entries na-4 .. na-1 above, the diagonal, then the nb below
+    const int m = nb;
+    w[0] = m == 0 ? wa[1] : m == 1 ? wa[2] : m == 2 ? wa[3] : m == 3 ? v : bl[0];
+    w[1] = m == 0 ? wa[2] : m == 1 ? wa[3] : m == 2 ? v : m == 3 ? bl[0] : bl[1];
+    w[2] = m == 0 ? wa[3] : m == 1 ? v : m == 2 ? bl[0] : m == 3 ? bl[1] : bl[2];
+    w[3] = m == 0 ? v : m == 1 ? bl[0] : m == 2 ? bl[1] : m == 3 ? bl[2] : bl[3];
+  } else {
+    w[0] = bl[0];
+    w[1] = bl[1];
+    w[2] = bl[2];
+    w[3] = bl[3];
+    for (int qb = qd + 1 + kStarBelow; qb < q1; qb += 4) {
+      double tv[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k)
-      if (qb + k < q1) {
-        const double lv = tv[k] / v;
-        LU[(size_t)(qb + k) * B + inst] = lv;
-        w[0] = w[1];
-        w[1] = w[2];
-        w[2] = w[3];
-        w[3] = lv;
-      }
+      for (int k = 0; k < 4; ++k)
+        tv[k] = A[(size_t)min(qb + k, q1 - 1) * B + inst];
+#pragma unroll
+      for (int k = 0; k < 4; ++k)
+        if (qb + k < q1) {
+          const double lv = tv[k] / v;
+          LU[(size_t)(qb + k) * B + inst] = lv;
+          w[0] = w[1];
+          w[1] = w[2];
+          w[2] = w[3];
+          w[3] = lv;
+        }
+    }
   }
   p0 = w[0];
   p1 = w[1];
@@ -745,6 +842,27 @@ __device__ __forceinline__ void star_column(const Pat &P, int t, int c,
     x[(size_t)c * B + inst] = y;
     yprev = y;
   }
+}
+
+template <int W, bool FUSE>
+__device__ __forceinline__ void star_column(const Pat &P, int t, int c,
+                                            const double *A, double *LU,
+                                            const double *b, double *x,
+                                            size_t B, int inst, double &p0,
+                                            double &p1, double &p2,
+                                            double &p3, double &yprev) {
+  int f[16];
+  load_record<W>(P.lu_rec + (size_t)t * 16, f);
+  const int nd = f[6];
+  if (nd <= 8)
+    star_body<8, W, FUSE>(P, f, c, A, LU, b, x, B, inst, p0, p1, p2, p3, yprev);
+  else if (nd <= 16)
+    star_body<16, W, FUSE>(P, f, c, A, LU, b, x, B, inst, p0, p1, p2, p3, yprev);
+  else if (nd <= 24)
+    star_body<24, W, FUSE>(P, f, c, A, LU, b, x, B, inst, p0, p1, p2, p3, yprev);
+  else
+    star_body<kStarMax, W, FUSE>(P, f, c, A, LU, b, x, B, inst, p0, p1, p2, p3,
+                                 yprev);
 }
 
 // Backward-substitution values of one row: y, U(i,i) and the two U terms.
@@ -815,15 +933,25 @@ __device__ __forceinline__ void fast_lu_chain(const Pat &P, int t0, int t1,
         int sr[kStampRec];
         load_record<W>(P.stamp_seg + (size_t)t * kStampRec, sr);
         if (sr[0]) {
-          if (((sflag >> 4) & 3) == 2)
-            tri_lu_segment_stamped<D, 2, STREAM>(sr, f[0], f[1], len, dslot,
-                                                 dcol, 2, P.geq, gv, LU, b, x,
-                                                 B, inst, p0, p1, p2, p3,
-                                                 yprev, badcol);
-          else
-            tri_lu_segment_stamped<D, -1, STREAM>(
-                sr, f[0], f[1], len, dslot, dcol, (sflag >> 4) & 3, P.geq, gv,
-                LU, b, x, B, inst, p0, p1, p2, p3, yprev, badcol);
+          const bool live = P.live ? P.live[inst] != 0 : true;
+          const int kc = (sflag >> 4) & 3;
+#define PVGLU_STAMPED(KC_, RES_, XH_)                                          \
+  tri_lu_segment_stamped<D, KC_, STREAM, RES_, XH_>(                           \
+      sr, f[0], f[1], len, dslot, dcol, kc, P.geq, P.ieq, P.xk, live, gv, LU,  \
+      b, x, B, inst, p0, p1, p2, p3, yprev, badcol)
+          if ((sr[6] & 1) && P.ieq && kc == 2) {
+            // node voltages are read only when a lane of the wavefront is on
+            // a homotopy stage (uniform choice of the variant)
+            if (__ballot(live && gv > 0) != 0)
+              PVGLU_STAMPED(2, true, true);
+            else
+              PVGLU_STAMPED(2, true, false);
+          } else if (kc == 2) {
+            PVGLU_STAMPED(2, false, true);
+          } else {
+            PVGLU_STAMPED(-1, false, true);
+          }
+#undef PVGLU_STAMPED
           if (badcol >= 0)
             record_status(P.status, PVGLU_E_SINGULAR, badcol, inst);
           t += len;

@@ -62,6 +62,14 @@ struct Pat {
   const int *stamp_seg;
   const double *geq;
   const double *gvs;
+  // Correction-form right-hand side formed on load (records with sr[6] bit
+  // 0): the module branch currents ieq [module][instance], the node voltages
+  // xk [row][instance] of the iteration (homotopy stages) and the batched
+  // Newton's instance activity flags (a frozen instance's right-hand side is
+  // zero).  Null when the launch has no such records.
+  const double *ieq;
+  const double *xk;
+  const int *live;
 };
 
 // Ints per kernel position of a stamping record: [0] 1 when the segment's A
@@ -69,7 +77,13 @@ struct Pat {
 // column, [2] its step per column; [3] module of the L-below entry, [4] its
 // step; [5] bits: 0/1/2/3 sign of the U-above / L-below / first diagonal /
 // second diagonal term negative, 4 the diagonal's first term is the U-above
-// module.  The entries' linear parts are zero.
+// module.  The entries' linear parts are zero.  [6] the correction-form
+// right-hand side of the segment's rows formed on load: bit 0 on, bit 1 the
+// row's first branch current is the U-above module's, bits 2/3 the first /
+// second term's sign negative (the row has exactly these two module
+// branches and no linear one, and the modules' other terminals are the
+// segment's previous and next nodes); [7] the row of the last column's
+// L-below entry (its next node).
 constexpr int kStampRec = 8;
 
 struct Lev {

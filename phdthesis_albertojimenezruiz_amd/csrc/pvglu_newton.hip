@@ -97,6 +97,12 @@ struct NewtonDev {
   std::vector<int> h_stamp_seg;
   int npart = 0;
   bool stamped = false; // the captured graphs stamp on load
+  // the rows whose correction-form right-hand side the stamped refactor does
+  // not form (the residual kernel's row list when it forms any) and whether
+  // the captured graphs use it
+  int *res_list = nullptr;
+  int nres = 0;
+  bool res_on_load = false;
   int fused = 1;        // stamping on load requested when captured
   hipStream_t s = nullptr;
   // one graph per iteration; the correction form alternates two (parity p
@@ -217,7 +223,9 @@ __global__ void __launch_bounds__(256)
 // right-hand side, so their correction is zero and x stays put after the
 // first replay that carries their last correction over.  Layout: one block
 // per kStampRows rows x up to 256 instances, as the stamping gathers; a
-// string row reads its two modules' branch currents only.
+// string row reads its two modules' branch currents only.  `rows` (may be
+// null: rows 0 .. n-1) lists the rows to compute: with the right-hand side
+// formed on load by the stamped refactor, only the rows it does not form.
 __global__ void __launch_bounds__(256)
     k_residual(int n, int B, const int *__restrict__ mptr,
                const int *__restrict__ mmod, const int *__restrict__ msgn,
@@ -226,7 +234,8 @@ __global__ void __launch_bounds__(256)
                const double *__restrict__ le, const int *__restrict__ pos,
                const int *__restrict__ neg, const double *__restrict__ ibr,
                const double *__restrict__ gvp, const double *__restrict__ x,
-               double *__restrict__ r, const int *active) {
+               double *__restrict__ r, const int *active,
+               const int *__restrict__ rows) {
   const int inst = (int)(blockIdx.y * blockDim.x + threadIdx.x);
   if (inst >= B)
     return;
@@ -236,32 +245,56 @@ __global__ void __launch_bounds__(256)
   auto node = [&](int k) {
     return k < 0 ? 0.0 : x[(size_t)k * B + inst];
   };
-  int mp[kStampRows + 1], lp[kStampRows + 1];
+  // the block's rows (n = the number of rows listed) and their ranges
+  int row[kStampRows], m0p[kStampRows], m1p[kStampRows], l0p[kStampRows],
+      l1p[kStampRows];
+  if (rows) {
 #pragma unroll
-  for (int q = 0; q <= kStampRows; ++q) {
-    mp[q] = mptr[min(r0 + q, n)];
-    lp[q] = lptr[min(r0 + q, n)];
+    for (int q = 0; q < kStampRows; ++q)
+      row[q] = rows[min(r0 + q, n - 1)];
+#pragma unroll
+    for (int q = 0; q < kStampRows; ++q) {
+      m0p[q] = mptr[row[q]];
+      m1p[q] = mptr[row[q] + 1];
+      l0p[q] = lptr[row[q]];
+      l1p[q] = lptr[row[q] + 1];
+    }
+  } else { // consecutive rows share their range bounds
+    int mb[kStampRows + 1], lb[kStampRows + 1];
+#pragma unroll
+    for (int q = 0; q <= kStampRows; ++q) {
+      mb[q] = mptr[min(r0 + q, n)];
+      lb[q] = lptr[min(r0 + q, n)];
+    }
+#pragma unroll
+    for (int q = 0; q < kStampRows; ++q) {
+      row[q] = min(r0 + q, n - 1);
+      m0p[q] = mb[q];
+      m1p[q] = mb[q + 1];
+      l0p[q] = lb[q];
+      l1p[q] = lb[q + 1];
+    }
   }
   if (!live) {
     for (int q = 0; q < kStampRows && r0 + q < n; ++q)
-      r[(size_t)(r0 + q) * B + inst] = 0.0;
+      r[(size_t)row[q] * B + inst] = 0.0;
     return;
   }
   bool simple = gv <= 0;
   bool anylin = false;
 #pragma unroll
   for (int q = 0; q < kStampRows; ++q) {
-    simple = simple && mp[q + 1] - mp[q] <= 2 && lp[q + 1] - lp[q] <= 1;
-    anylin = anylin || lp[q + 1] > lp[q];
+    simple = simple && m1p[q] - m0p[q] <= 2 && l1p[q] - l0p[q] <= 1;
+    anylin = anylin || l1p[q] > l0p[q];
   }
   if (simple) {
     // every load first (absent terms load a valid stand-in), then combine
     double i0[kStampRows], i1[kStampRows], xa[kStampRows], xb[kStampRows];
 #pragma unroll
     for (int q = 0; q < kStampRows; ++q) {
-      const int cm = mp[q + 1] - mp[q];
-      const int m0 = cm > 0 ? mmod[mp[q]] : 0;
-      const int m1 = cm > 1 ? mmod[mp[q] + 1] : m0;
+      const int cm = m1p[q] - m0p[q];
+      const int m0 = cm > 0 ? mmod[m0p[q]] : 0;
+      const int m1 = cm > 1 ? mmod[m0p[q] + 1] : m0;
       i0[q] = ibr[(size_t)m0 * B + inst];
       i1[q] = ibr[(size_t)m1 * B + inst];
       xa[q] = xb[q] = 0.0;
@@ -269,45 +302,43 @@ __global__ void __launch_bounds__(256)
     if (anylin) { // combiner and bus rows: node voltages of the linear branch
 #pragma unroll
       for (int q = 0; q < kStampRows; ++q)
-        if (lp[q + 1] > lp[q]) {
-          xa[q] = node(lca[lp[q]]);
-          const int cb = lcb[lp[q]];
-          xb[q] = cb >= 0 ? node(cb) : le[lp[q]];
+        if (l1p[q] > l0p[q]) {
+          xa[q] = node(lca[l0p[q]]);
+          const int cb = lcb[l0p[q]];
+          xb[q] = cb >= 0 ? node(cb) : le[l0p[q]];
         }
     }
 #pragma unroll
     for (int q = 0; q < kStampRows; ++q) {
-      const int row = r0 + q;
-      if (row >= n)
+      if (r0 + q >= n)
         break;
       double acc = 0.0;
-      const int cm = mp[q + 1] - mp[q];
+      const int cm = m1p[q] - m0p[q];
       if (cm > 0)
-        acc = acc - (double)msgn[mp[q]] * i0[q];
+        acc = acc - (double)msgn[m0p[q]] * i0[q];
       if (cm > 1)
-        acc = acc - (double)msgn[mp[q] + 1] * i1[q];
-      if (lp[q + 1] > lp[q])
-        acc = acc - lg[lp[q]] * (xa[q] - xb[q]);
-      r[(size_t)row * B + inst] = acc;
+        acc = acc - (double)msgn[m0p[q] + 1] * i1[q];
+      if (l1p[q] > l0p[q])
+        acc = acc - lg[l0p[q]] * (xa[q] - xb[q]);
+      r[(size_t)row[q] * B + inst] = acc;
     }
     return;
   }
   for (int q = 0; q < kStampRows && r0 + q < n; ++q) {
-    const int row = r0 + q;
     double acc = 0.0;
-    for (int k = mp[q]; k < mp[q + 1]; ++k) {
+    for (int k = m0p[q]; k < m1p[q]; ++k) {
       const int e = mmod[k];
       double ie = ibr[(size_t)e * B + inst];
       if (gv > 0)
         ie = ie + gv * (node(pos[e]) - node(neg[e]));
       acc = acc - (double)msgn[k] * ie;
     }
-    for (int k = lp[q]; k < lp[q + 1]; ++k) {
+    for (int k = l0p[q]; k < l1p[q]; ++k) {
       const int cb = lcb[k];
       const double xb = cb >= 0 ? node(cb) : le[k];
       acc = acc - lg[k] * (node(lca[k]) - xb);
     }
-    r[(size_t)row * B + inst] = acc;
+    r[(size_t)row[q] * B + inst] = acc;
   }
 }
 
@@ -841,6 +872,64 @@ int newton_setup(pvglu_pv *pv) {
     }
     l_ptr.push_back((int)l_ca.size());
   }
+  // Right-hand side on load (record [6]): every row of the segment has
+  // exactly the two module branches of its column's U-above and L-below
+  // entries and no linear branch, in a branch order and with signs that are
+  // the same in every row.
+  for (int t = 0; t < n; ++t) {
+    int *r = &sseg[(size_t)t * kStampRec];
+    if (!r[0])
+      continue;
+    const int *sg = &P->fast_lu_seg[(size_t)t * 4];
+    const int len = sg[0], dcol = sg[2];
+    const int col0 = P->fast_lu_rec[(size_t)t * 16 + 1];
+    // a chained string (column i's L-below module is column i+1's U-above
+    // one) whose forwarded entry is the L entry (the kernel's fixed case)
+    const bool chained = r[2] == r[4] && r[1] + r[2] == r[3];
+    if (!chained || ((sg[3] >> 4) & 3) != 2) {
+      r[6] = 0;
+      continue;
+    }
+    int bits = -1;
+    for (int i = 0; i < len; ++i) {
+      const int row = col0 + i * dcol;
+      const int mu = r[1] + i * r[2], ml = r[3] + i * r[4];
+      const int b0 = b_ptr[row];
+      if (b_ptr[row + 1] - b0 != 2 || l_ptr[row + 1] != l_ptr[row]) {
+        bits = -1;
+        break;
+      }
+      const int m0 = b_mod[b0], m1 = b_mod[b0 + 1];
+      if (!((m0 == mu && m1 == ml) || (m0 == ml && m1 == mu))) {
+        bits = -1;
+        break;
+      }
+      // the modules' other terminals: the rows of the U-above and L-below
+      // entries (node voltages of homotopy stages), the previous node of the
+      // segment and, but in its last column, the next one (the last
+      // column's is recorded in [7])
+      auto joins = [&](int e, int a, int c) {
+        return (pos[e] == a && neg[e] == c) || (pos[e] == c && neg[e] == a);
+      };
+      const int q = P->fast_lu_rec[(size_t)t * 16] + i * sg[1];
+      const int rprev = P->ri[q], rnext = P->ri[q + 2];
+      if (rprev != row - dcol || (i + 1 < len && rnext != row + dcol) ||
+          !joins(mu, rprev, row) || !joins(ml, row, rnext)) {
+        bits = -1;
+        break;
+      }
+      if (i + 1 == len)
+        r[7] = rnext;
+      const int rb = 1 | (m0 == mu ? 2 : 0) | (b_sgn[b0] < 0 ? 4 : 0) |
+                     (b_sgn[b0 + 1] < 0 ? 8 : 0);
+      if (i > 0 && rb != bits) {
+        bits = -1;
+        break;
+      }
+      bits = rb;
+    }
+    r[6] = bits > 0 ? bits : 0;
+  }
   auto *d = new NewtonDev();
   pv->nd = d;
   auto &A = d->allocs;
@@ -860,6 +949,7 @@ int newton_setup(pvglu_pv *pv) {
                     std::vector<double>(kLadder, kLadder + 12))) ||
       (st = dupload(A, &d->lk, std::vector<double>(pv->nmod, 0.0))) ||
       (st = dupload(A, &d->stamp_seg, sseg)) ||
+      (st = dupload(A, &d->res_list, std::vector<int>(std::max(n, 1), 0))) ||
       (st = dupload(A, &d->a_part, std::vector<int>(std::max(nnz, 1), 0))))
     return st;
   d->h_stamp_seg.swap(sseg);
@@ -966,6 +1056,8 @@ inline double noise_of(int form) {
 int prepare_stamping(pvglu_pv *pv, int B) {
   NewtonDev *d = pv->nd;
   d->stamped = false;
+  d->res_on_load = false;
+  d->nres = 0;
   d->npart = 0;
   if (!pv->fused_stamping || d->form != PVGLU_NEWTON_CORRECTION)
     return PVGLU_OK;
@@ -977,19 +1069,40 @@ int prepare_stamping(pvglu_pv *pv, int B) {
   if (levels.empty())
     return PVGLU_OK;
   std::vector<char> covered(P->nnz, 0);
+  std::vector<int> rrow(pv->n, 0);
+  bool anyres = false;
   const KernelLevels &K = P->k_lu;
   for (int lv : levels)
     for (int ch = K.level_chain_ptr[lv]; ch < K.level_chain_ptr[lv + 1]; ++ch)
       for (int t = K.chain_col_ptr[ch]; t < K.chain_col_ptr[ch + 1]; ++t) {
-        if (!d->h_stamp_seg[(size_t)t * kStampRec])
+        const int *r = &d->h_stamp_seg[(size_t)t * kStampRec];
+        if (!r[0])
           continue;
         const int len = P->fast_lu_seg[(size_t)t * 4];
         const int dslot = P->fast_lu_seg[(size_t)t * 4 + 1];
+        const int dcol = P->fast_lu_seg[(size_t)t * 4 + 2];
         const int slot0 = P->fast_lu_rec[(size_t)t * 16];
-        for (int i = 0; i < len; ++i)
+        const int col0 = P->fast_lu_rec[(size_t)t * 16 + 1];
+        for (int i = 0; i < len; ++i) {
           for (int k = 0; k < 3; ++k)
             covered[slot0 + i * dslot + k] = 1;
+          if (r[6] & 1) {
+            rrow[col0 + i * dcol] = 1;
+            anyres = true;
+          }
+        }
       }
+  if (anyres) {
+    std::vector<int> rest;
+    for (int r = 0; r < pv->n; ++r)
+      if (!rrow[r])
+        rest.push_back(r);
+    if (!rest.empty())
+      HIP_TRY(hipMemcpy(d->res_list, rest.data(), sizeof(int) * rest.size(),
+                        hipMemcpyHostToDevice));
+    d->nres = (int)rest.size();
+  }
+  d->res_on_load = anyres;
   std::vector<int> part;
   for (int q = 0; q < P->nnz; ++q)
     if (!covered[q])
@@ -1024,8 +1137,10 @@ int launch_iteration(pvglu_pv *pv, int B, int parity) {
                                     d->skip ? d->active : nullptr)))
       return st;
   } else {
+    const bool res = d->stamped && d->res_on_load;
+    const int nrow = res ? d->nres : n;
     const RowGrid ga = stamp_grid(d->stamped ? d->npart : nnz, B),
-                  gb = stamp_grid(n, B);
+                  gb = stamp_grid(nrow, B);
     if (!d->stamped)
       hipLaunchKernelGGL(k_stamp_a, ga.grid, ga.block, 0, d->s, nnz, B,
                          d->base_a, d->a_ptr, d->a_mod, d->a_sgn, d->geq,
@@ -1040,15 +1155,20 @@ int launch_iteration(pvglu_pv *pv, int B, int parity) {
     hipLaunchKernelGGL(k_add, grid_for((cnt + 3) / 4), dim3(256), 0, d->s, cnt,
                        d->X[parity], d->dx, d->X[1 - parity], B,
                        d->skip ? d->active : nullptr);
-    hipLaunchKernelGGL(k_residual, gb.grid, gb.block, 0, d->s, n, B, d->b_ptr,
-                       d->b_mod, d->b_sgn, d->l_ptr, d->l_ca, d->l_cb, d->l_g,
-                       d->l_e, d->pos, d->neg, d->ieq, d->gvs, d->X[1 - parity],
-                       d->b, d->active);
+    if (nrow > 0)
+      hipLaunchKernelGGL(k_residual, gb.grid, gb.block, 0, d->s, nrow, B,
+                         d->b_ptr, d->b_mod, d->b_sgn, d->l_ptr, d->l_ca,
+                         d->l_cb, d->l_g, d->l_e, d->pos, d->neg, d->ieq,
+                         d->gvs, d->X[1 - parity], d->b, d->active,
+                         res ? d->res_list : nullptr);
     HIP_TRY(hipGetLastError());
     // a tile of frozen instances is skipped: k_add cleared their corrections
     const int *act = d->skip ? d->active : nullptr;
     if (d->stamped) {
-      const StampOnLoad so{d->stamp_seg, d->geq, d->gvs};
+      const StampOnLoad so{d->stamp_seg, d->geq, d->gvs,
+                           d->res_on_load ? d->ieq : nullptr,
+                           d->res_on_load ? d->X[1 - parity] : nullptr,
+                           d->res_on_load ? d->active : nullptr};
       if ((st = refactor_solve_stamped(pv->plan, so, d->A, d->LU, d->b, d->dx,
                                        B, d->s, act)))
         return st;
