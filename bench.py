@@ -366,7 +366,13 @@ def main():
     G = np.stack([plant.irradiance(s, 0.0) for s in range(B)], axis=1)
     if args.profile_newton > 0:
         # a few replays from the cold start: every instance active, the
-        # kernels of one Newton iteration as the solve runs them
+        # kernels of one Newton iteration as the solve runs them, on the
+        # schedule given (counter collection serializes the autotuner's
+        # timing runs, so its pick under a counter pass is not the loop's)
+        if args.schedule != "auto":
+            v, w, g = args.schedule.split(":")
+            plan.set_schedule(SCHED_NAMES[v], int(w))
+            plan.set_stream_groups(int(g))
         plant.newton_batched(G, tol=1e-12, max_iters=2, ladder=False, raise_noconv=False)
         t0 = time.perf_counter()
         plant.newton_batched(G, tol=1e-12, max_iters=args.profile_newton, ladder=False,
