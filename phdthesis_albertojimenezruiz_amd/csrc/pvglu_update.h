@@ -49,8 +49,8 @@ __host__ __device__ inline double fixed_to_delta(unsigned long long s) {
 // (TW = the power of two >= B, at most 256) by RP = 256/TW element rows and
 // covers RP * kUpdPerThread consecutive elements, row r taking e0 + r,
 // e0 + r + RP, ...; the loads of kUpdBatch elements are issued together.
-constexpr int kUpdPerThread = 8;
-constexpr int kUpdBatch = 4;
+constexpr int kUpdPerThread = 16;
+constexpr int kUpdBatch = 2;
 __host__ __device__ inline int upd_tile_log2(int B) {
   return B >= 256 ? 8 : (B <= 1 ? 0 : 32 - __builtin_clz((unsigned)(B - 1)));
 }
