@@ -8,8 +8,8 @@ iteration's linear algebra).  Default workload (the batched config):
 (26,920 FP64 unknowns, 80,680 nonzeros under AMD ordering, zero fill), all
 instances sharing one pattern.  The systems are the plant's companion-model
 Jacobians stamped on the GPU at the converged Newton state of each scenario
-(batched device Newton to 1e-10 V), so values are device-resident when the
-timed region starts.  Beside it on the same line:
+(batched device Newton, correction form, to 1e-12 V), so values are
+device-resident when the timed region starts.  Beside it on the same line:
   * plain: the single-instance GPU path on the same plant (batch 1, the
     library's default schedule), with the reference's launch structure (one
     kernel per level, one lane per chain) timed beside it;
