@@ -112,8 +112,10 @@ __global__ void __launch_bounds__(256)
                     double *__restrict__ v, unsigned long long *delta_sum,
                     unsigned long long *delta_max, const int *active,
                     int *status) {
-  update_elements(ne, B, pos, neg, x, nullptr, v, delta_sum, delta_max, active,
-                  status, [](int, int, double, double) {});
+  update_elements<false>(
+      ne, B, pos, neg, x, nullptr, v, delta_sum, delta_max, active, status,
+      [](int, int) { return NoPre{}; }, [](int, int, double, double, NoPre) {},
+      nullptr, nullptr);
 }
 
 // The fixed-point sums of k_newton_update converted to volts in place.

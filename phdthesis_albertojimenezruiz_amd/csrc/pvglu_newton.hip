@@ -71,6 +71,12 @@ struct NewtonDev {
   // conductance l_g (+ at the row of ca, - at the row of cb)
   int *l_ptr = nullptr, *l_ca = nullptr, *l_cb = nullptr;
   double *l_g = nullptr, *l_e = nullptr;
+  // correction form: the element terminals with the kOwnTag of the element
+  // that writes each node's next voltage (upos, uneg) and the nodes no
+  // element terminal reaches (left[nleft]), which the update's extra blocks
+  // write
+  int *upos = nullptr, *uneg = nullptr, *left = nullptr;
+  int nleft = 0;
   // instance-side
   // ieq: the companion current source f - G v (direct form) or the branch
   // current at the new node voltages f + G (V - v) (correction form)
@@ -84,6 +90,10 @@ struct NewtonDev {
   unsigned long long *dsum = nullptr, *dmax = nullptr;
   int *stage = nullptr, *active = nullptr, *stall = nullptr, *iters = nullptr,
       *remaining = nullptr;
+  // set by the control step that freezes an instance, cleared by the next
+  // one: the update in between copies the instance's final node voltages
+  // into the other node buffer
+  int *justf = nullptr;
   int nstages = 12; // stages of the current solve (12 = ladder, 1 = last only)
   double tol = 1e-10;    // captured into the graph; re-captured when changed
   int form = PVGLU_NEWTON_CORRECTION; // form the graphs were captured for
@@ -106,7 +116,7 @@ struct NewtonDev {
   int fused = 1;        // stamping on load requested when captured
   hipStream_t s = nullptr;
   // one graph per iteration; the correction form alternates two (parity p
-  // reads node voltages X[p] and writes X[1 - p])
+  // reads node voltages X[p] and its update writes X[1 - p])
   hipGraphExec_t exec = nullptr, exec2 = nullptr;
   hipGraph_t graph = nullptr, graph2 = nullptr;
   std::vector<void *> allocs;
@@ -184,7 +194,18 @@ __global__ void k_eval(int nm, int B, const double *__restrict__ Girr,
 // Correction form: x + dx are the new node voltages (x = the previous ones,
 // dx = the solve's correction) and ieq <- f + G (V - v), the linearized
 // branch current at those node voltages, which the next iteration's residual
-// gathers (V - v = 0 exactly when the step was not damped).
+// gathers (V - v = 0 exactly when the step was not damped).  The update also
+// writes those node voltages, x + dx, to xo for the next iteration: at the
+// element terminals tagged kOwnTag (pos, neg are the tagged lists) and, in
+// the blocks past the nblk element blocks, at the nodes left[0..nleft) that
+// no element reaches.  The additions are those the next iteration would
+// otherwise make over the whole vector, so the iterates are the same bits.
+// The module operands of an element's evaluation, loaded with its update.
+struct ModuleOperands {
+  double g, rsh, lk;
+};
+
+template <bool CORR>
 __global__ void __launch_bounds__(256)
     k_update_eval(int ne, int B, const int *__restrict__ pos,
                   const int *__restrict__ neg, const double *__restrict__ x,
@@ -194,25 +215,48 @@ __global__ void __launch_bounds__(256)
                   const double *__restrict__ Girr,
                   const double *__restrict__ rsh,
                   const double *__restrict__ lk, double *__restrict__ geq,
-                  double *__restrict__ ieq, int *remaining) {
+                  double *__restrict__ ieq, int *remaining,
+                  double *__restrict__ xo, const int *__restrict__ justf,
+                  int nblk, const int *__restrict__ left, int nleft) {
   // the control kernel after this one counts the unfinished instances
   if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
     *remaining = 0;
-  update_elements(ne, B, pos, neg, x, dx, v, dsum, dmax, active, status,
-                  [&](int e, int in, double V, double Vx) {
-                    const size_t gid = (size_t)e * B + in;
-                    const ModuleEval me = module_current_lk(
-                        V, kIphPerIrradiance * Girr[gid], rsh[e], lk[e]);
-                    const double f = -me.I, g = -me.dIdV;
-                    geq[gid] = g;
-                    ieq[gid] = dx ? f + g * (Vx - V) : f - g * V;
-                  });
+  if (CORR && (int)blockIdx.x >= nblk) { // nodes without an element terminal
+    const int lt = upd_tile_log2(B), rp = 256 >> lt;
+    const int in = (int)(blockIdx.y << lt) + (int)(threadIdx.x & ((1u << lt) - 1u));
+    if (in >= B)
+      return;
+    const bool live = active[in] != 0, copy = !live && justf[in] != 0;
+    if (!live && !copy)
+      return;
+    const int k0 = ((int)blockIdx.x - nblk) * rp * kUpdPerThread;
+    const int k1 = min(nleft, k0 + rp * kUpdPerThread);
+    for (int q = k0 + (int)(threadIdx.x >> lt); q < k1; q += rp) {
+      const size_t i = (size_t)left[q] * B + in;
+      xo[i] = live ? x[i] + dx[i] : x[i];
+    }
+    return;
+  }
+  update_elements<CORR>(
+      ne, B, pos, neg, x, dx, v, dsum, dmax, active, status,
+      [&](int e, int in) {
+        return ModuleOperands{Girr[(size_t)e * B + in], rsh[e], lk[e]};
+      },
+      [&](int e, int in, double V, double Vx, const ModuleOperands &m) {
+        const size_t gid = (size_t)e * B + in;
+        const ModuleEval me =
+            module_current_lk(V, kIphPerIrradiance * m.g, m.rsh, m.lk);
+        const double f = -me.I, g = -me.dIdV;
+        geq[gid] = g;
+        ieq[gid] = CORR ? f + g * (Vx - V) : f - g * V;
+      },
+      xo, justf);
 }
 
 // Correction-form right-hand side.  Per permuted row (equation of one node)
 // and instance: minus the sum of the currents leaving the node through its
-// branches, at the node voltages x (this iteration's, k_add having formed
-// them as the previous iterate plus its correction):
+// branches, at the node voltages x (this iteration's: the previous iterate
+// plus its correction, written by the previous update):
 //   modules  ibr (from k_update_eval), + g_virtual (x[pos] - x[neg]) on
 //            homotopy stages;
 //   linear   l_g (x[ca] - x[cb]), cb < 0 meaning a source of EMF l_e (the
@@ -220,8 +264,8 @@ __global__ void __launch_bounds__(256)
 // Each difference of node voltages is formed before it is scaled, so a
 // string's ~600 V node voltages cancel exactly and the residual carries the
 // rounding of the branch currents only.  Frozen instances get a zero
-// right-hand side, so their correction is zero and x stays put after the
-// first replay that carries their last correction over.  Layout: one block
+// right-hand side (their correction is not used: the update skips them).
+// Layout: one block
 // per kStampRows rows x up to 256 instances, as the stamping gathers; a
 // string row reads its two modules' branch currents only.  `rows` (may be
 // null: rows 0 .. n-1) lists the rows to compute: with the right-hand side
@@ -339,45 +383,6 @@ __global__ void __launch_bounds__(256)
       acc = acc - lg[k] * (node(lca[k]) - xb);
     }
     r[(size_t)row[q] * B + inst] = acc;
-  }
-}
-
-// x = a + b elementwise (the correction form's node voltages: the previous
-// iterate plus its correction), four values per thread as two 16-byte loads
-// of each operand (buffers from hipMalloc, 256-byte aligned).  With `active`
-// (flags per instance, values [row][instance] with B instances), the
-// correction b of a frozen instance is cleared once added: a frozen
-// instance's right-hand side is zero, so its correction is zero from then
-// on, and a refactor + solve that skips a tile of frozen instances must find
-// it so.
-__global__ void k_add(long count, const double *__restrict__ a,
-                      double *__restrict__ b, double *__restrict__ x, int B,
-                      const int *__restrict__ active) {
-  const long i = 4 * ((long)blockIdx.x * blockDim.x + threadIdx.x);
-  if (i + 3 < count) {
-    // streaming hints: the operands and the sum are next read after the
-    // refactor + solve has streamed far more than the caches hold
-    typedef double d2 __attribute__((ext_vector_type(2)));
-    const d2 a0 = __builtin_nontemporal_load((const d2 *)(a + i));
-    const d2 a1 = __builtin_nontemporal_load((const d2 *)(a + i + 2));
-    const d2 b0 = __builtin_nontemporal_load((const d2 *)(b + i));
-    const d2 b1 = __builtin_nontemporal_load((const d2 *)(b + i + 2));
-    __builtin_nontemporal_store(a0 + b0, (d2 *)(x + i));
-    __builtin_nontemporal_store(a1 + b1, (d2 *)(x + i + 2));
-    if (active) {
-      int k = (int)(i % B);
-      for (int j = 0; j < 4; ++j) {
-        if (!active[k])
-          b[i + j] = 0.0;
-        k = k + 1 == B ? 0 : k + 1;
-      }
-    }
-  } else {
-    for (long j = i; j < count; ++j) {
-      x[j] = a[j] + b[j];
-      if (active && !active[(int)(j % B)])
-        b[j] = 0.0;
-    }
   }
 }
 
@@ -528,35 +533,32 @@ __global__ void k_zero(long count, double *p) {
 }
 
 // Compaction of a [row][instance] array (rows x Bc, instance-minor) to the
-// instances sel[0..Bn) (rows x Bn): out[r][j] = in[r][sel[j]] (+ add[r][sel[j]]
-// when add is given: the correction form's pending correction folded into the
-// node voltages).
+// instances sel[0..Bn) (rows x Bn): out[r][j] = in[r][sel[j]].
 template <typename T>
 __global__ void k_gather(long rows, int Bc, int Bn, const int *__restrict__ sel,
-                         const T *__restrict__ in, const T *__restrict__ add,
-                         T *__restrict__ out) {
+                         const T *__restrict__ in, T *__restrict__ out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= rows * Bn)
     return;
   const long r = i / Bn;
   const long src = r * Bc + sel[i - r * Bn];
-  out[i] = add ? (T)(in[src] + add[src]) : in[src];
+  out[i] = in[src];
 }
 
 // The inverse for the instances leaving a compacted batch: out[r][to[k]] =
-// in[r][from[k]] (+ add), out with Bh instances per row (the caller's batch).
+// in[r][from[k]], out with Bh instances per row (the caller's batch).
 template <typename T>
 __global__ void k_scatter(long rows, int Bc, int Bh, int cnt,
                           const int *__restrict__ from,
                           const int *__restrict__ to, const T *__restrict__ in,
-                          const T *__restrict__ add, T *__restrict__ out) {
+                          T *__restrict__ out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= rows * cnt)
     return;
   const long r = i / cnt;
   const long k = i - r * cnt;
   const long src = r * Bc + from[k];
-  out[r * Bh + to[k]] = add ? (T)(in[src] + add[src]) : in[src];
+  out[r * Bh + to[k]] = in[src];
 }
 
 __global__ void k_power_grad_bus(int nbus, int B, const int *__restrict__ bus_c,
@@ -618,10 +620,11 @@ __global__ void k_fill_double(double *p, int n, double v) {
 __global__ void k_newton_init(int B, double gv0, int *stage, double *gvs,
                               int *active, int *stall, int *iters,
                               double *lastmax, unsigned long long *dsum,
-                              unsigned long long *dmax) {
+                              unsigned long long *dmax, int *justf) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= B)
     return;
+  justf[k] = 0;
   dsum[k] = 0; // the update accumulates into these; control clears them
   dmax[k] = 0;
   stage[k] = 0;
@@ -644,10 +647,16 @@ __global__ void k_newton_control(int B, int nstages,
                                  int stall_max, unsigned long long *dsum,
                                  unsigned long long *dmax, int *stage,
                                  double *gvs, int *active, int *stall,
-                                 int *iters, double *lastmax, int *remaining) {
+                                 int *iters, double *lastmax, int *remaining,
+                                 int *justf) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= B || !active[k])
+  if (k >= B)
     return;
+  if (!active[k]) {
+    if (justf) // the update since the freeze has copied its node voltages
+      justf[k] = 0;
+    return;
+  }
   iters[k]++;
   const int st = stage[k];
   const double m = __longlong_as_double((long long)dmax[k]);
@@ -665,11 +674,15 @@ __global__ void k_newton_control(int B, int nstages,
   }
   if (!(m > tol)) { // converged (NaN is caught by the status word)
     active[k] = 0;
+    if (justf)
+      justf[k] = 1;
     return;
   }
   if (m <= noise) {
     if (++stall[k] >= stall_max) {
       active[k] = 0;
+      if (justf)
+        justf[k] = 1;
       return;
     }
   } else {
@@ -837,6 +850,28 @@ int newton_setup(pvglu_pv *pv) {
     rpos[e] = pv->mod_pos[e] >= 0 ? rowmap[pv->mod_pos[e]] : -1;
     rneg[e] = pv->mod_neg[e] >= 0 ? rowmap[pv->mod_neg[e]] : -1;
   }
+  // the correction form's update writes each node's next voltage at the
+  // first element terminal (in element order) on the node; the nodes that no
+  // terminal reaches (the DC buses behind their cables) are listed
+  if (n >= kOwnTag)
+    return fail(PVGLU_E_ARG, "batched Newton: %d unknowns (at most 2^30 - 1)", n);
+  std::vector<int> upos(pos), uneg(neg), left;
+  {
+    std::vector<char> owned(n, 0);
+    for (int e = 0; e < nm; e++) {
+      if (pos[e] >= 0 && !owned[pos[e]]) {
+        owned[pos[e]] = 1;
+        upos[e] |= kOwnTag;
+      }
+      if (neg[e] >= 0 && !owned[neg[e]]) {
+        owned[neg[e]] = 1;
+        uneg[e] |= kOwnTag;
+      }
+    }
+    for (int k = 0; k < n; k++)
+      if (!owned[k])
+        left.push_back(k);
+  }
   std::vector<int> cra(pv->ncab), crb(pv->ncab), cca(pv->ncab), ccb(pv->ncab),
       busc(pv->I);
   for (int c = 0; c < pv->ncab; c++) {
@@ -939,6 +974,8 @@ int newton_setup(pvglu_pv *pv) {
       (st = dupload(A, &d->a_sgn, a_sgn)) || (st = dupload(A, &d->b_ptr, b_ptr)) ||
       (st = dupload(A, &d->b_mod, b_mod)) || (st = dupload(A, &d->b_sgn, b_sgn)) ||
       (st = dupload(A, &d->pos, pos)) || (st = dupload(A, &d->neg, neg)) ||
+      (st = dupload(A, &d->upos, upos)) || (st = dupload(A, &d->uneg, uneg)) ||
+      (st = dupload(A, &d->left, left)) ||
       (st = dupload(A, &d->rpos, rpos)) || (st = dupload(A, &d->rneg, rneg)) ||
       (st = dupload(A, &d->cab_ra, cra)) || (st = dupload(A, &d->cab_rb, crb)) ||
       (st = dupload(A, &d->cab_ca, cca)) || (st = dupload(A, &d->cab_cb, ccb)) ||
@@ -953,6 +990,7 @@ int newton_setup(pvglu_pv *pv) {
       (st = dupload(A, &d->a_part, std::vector<int>(std::max(nnz, 1), 0))))
     return st;
   d->h_stamp_seg.swap(sseg);
+  d->nleft = (int)left.size();
   HIP_TRY(hipStreamCreate(&d->s));
   if (pv->nmod > 0) {
     hipLaunchKernelGGL(k_module_lk, dim3((pv->nmod + 255) / 256), dim3(256), 0,
@@ -1005,7 +1043,7 @@ int newton_alloc_instances(pvglu_pv *pv, int B) {
       (st = dalloc(I, &d->gvs, (size_t)B)) || (st = dalloc(I, &d->lastmax, (size_t)B)) ||
       (st = dalloc(I, &d->stage, (size_t)B)) || (st = dalloc(I, &d->active, (size_t)B)) ||
       (st = dalloc(I, &d->stall, (size_t)B)) || (st = dalloc(I, &d->iters, (size_t)B)) ||
-      (st = dalloc(I, &d->remaining, 1)))
+      (st = dalloc(I, &d->justf, (size_t)B)) || (st = dalloc(I, &d->remaining, 1)))
     return st;
   d->B = B;
   return PVGLU_OK;
@@ -1117,9 +1155,9 @@ int prepare_stamping(pvglu_pv *pv, int B) {
 
 // One Newton iteration on d->s.  Direct form: stamping of A and b, solve for
 // the node voltages x.  Correction form (parity p): stamping of A, the
-// branch-current residual at X[p] + dx (writing those node voltages to
-// X[1 - p]), solve for the correction dx.  Then the fused update +
-// evaluation and the per-instance control.
+// branch-current residual at the node voltages X[p], solve for the
+// correction dx; the update writes X[p] + dx to X[1 - p].  Then the fused
+// update + evaluation and the per-instance control.
 int launch_iteration(pvglu_pv *pv, int B, int parity) {
   NewtonDev *d = pv->nd;
   const bool corr = d->form == PVGLU_NEWTON_CORRECTION;
@@ -1149,25 +1187,22 @@ int launch_iteration(pvglu_pv *pv, int B, int parity) {
       hipLaunchKernelGGL(k_stamp_a_list, ga.grid, ga.block, 0, d->s, d->npart,
                          d->a_part, B, d->base_a, d->a_ptr, d->a_mod,
                          d->a_sgn, d->geq, d->gvs, d->A, d->active);
-    // x of this iteration = the previous one plus its correction (every
-    // instance: a frozen one's correction is zero after its first replay)
-    const long cnt = (long)n * B;
-    hipLaunchKernelGGL(k_add, grid_for((cnt + 3) / 4), dim3(256), 0, d->s, cnt,
-                       d->X[parity], d->dx, d->X[1 - parity], B,
-                       d->skip ? d->active : nullptr);
+    // this iteration's node voltages X[parity] were written by the previous
+    // update (or zeroed before the first replay)
     if (nrow > 0)
       hipLaunchKernelGGL(k_residual, gb.grid, gb.block, 0, d->s, nrow, B,
                          d->b_ptr, d->b_mod, d->b_sgn, d->l_ptr, d->l_ca,
                          d->l_cb, d->l_g, d->l_e, d->pos, d->neg, d->ieq,
-                         d->gvs, d->X[1 - parity], d->b, d->active,
+                         d->gvs, d->X[parity], d->b, d->active,
                          res ? d->res_list : nullptr);
     HIP_TRY(hipGetLastError());
-    // a tile of frozen instances is skipped: k_add cleared their corrections
+    // a tile of frozen instances is skipped (the update does not read their
+    // corrections)
     const int *act = d->skip ? d->active : nullptr;
     if (d->stamped) {
       const StampOnLoad so{d->stamp_seg, d->geq, d->gvs,
                            d->res_on_load ? d->ieq : nullptr,
-                           d->res_on_load ? d->X[1 - parity] : nullptr,
+                           d->res_on_load ? d->X[parity] : nullptr,
                            d->res_on_load ? d->active : nullptr};
       if ((st = refactor_solve_stamped(pv->plan, so, d->A, d->LU, d->b, d->dx,
                                        B, d->s, act)))
@@ -1176,7 +1211,7 @@ int launch_iteration(pvglu_pv *pv, int B, int parity) {
                                            B, d->s, act))) {
       return st;
     }
-    xnew = d->X[1 - parity];
+    xnew = d->X[parity];
     dx = d->dx;
   }
   int *status = device_status(pv->plan);
@@ -1184,19 +1219,22 @@ int launch_iteration(pvglu_pv *pv, int B, int parity) {
     return fail(PVGLU_E_STATE, "batched Newton: plan has no device state");
   const int nm = pv->nmod, lt = upd_tile_log2(B);
   const int per_block = (256 >> lt) * kUpdPerThread;
+  const int nblk = (nm + per_block - 1) / per_block,
+            nlb = corr ? (d->nleft + per_block - 1) / per_block : 0;
   // dsum / dmax were cleared by k_newton_init or the previous control kernel
-  hipLaunchKernelGGL(k_update_eval,
-                     dim3((nm + per_block - 1) / per_block,
-                          (B + (1 << lt) - 1) >> lt),
-                     dim3(256), 0, d->s, nm, B, d->pos, d->neg, xnew, dx, d->v,
-                     d->dsum, d->dmax, d->active, status, d->G, d->rsh, d->lk,
-                     d->geq, d->ieq, d->remaining);
+  hipLaunchKernelGGL(corr ? k_update_eval<true> : k_update_eval<false>,
+                     dim3(nblk + nlb, (B + (1 << lt) - 1) >> lt),
+                     dim3(256), 0, d->s, nm, B, corr ? d->upos : d->pos,
+                     corr ? d->uneg : d->neg, xnew, dx, d->v, d->dsum, d->dmax,
+                     d->active, status, d->G, d->rsh, d->lk, d->geq, d->ieq,
+                     d->remaining, corr ? d->X[1 - parity] : nullptr, d->justf,
+                     nblk, d->left, d->nleft);
   const double *ladder = d->ladder + (12 - d->nstages);
   hipLaunchKernelGGL(k_newton_control, grid_for(B), dim3(256), 0, d->s, B,
                      d->nstages, ladder, 1e-6 * pv->n, d->tol,
                      noise_of(d->form), kStallMax, d->dsum, d->dmax, d->stage,
                      d->gvs, d->active, d->stall, d->iters, d->lastmax,
-                     d->remaining);
+                     d->remaining, corr ? d->justf : nullptr);
   HIP_TRY(hipGetLastError());
   return PVGLU_OK;
 }
@@ -1244,10 +1282,10 @@ int build_graph(pvglu_pv *pv, int B) {
 // three quarters of the working batch is still active, the finished
 // instances' results go to their columns of the caller's arrays (the "home"
 // set: G, v, x, iters, lastmax at batch B) and the active ones are gathered
-// into a denser working batch in one of two alternate sets, the
-// correction form's pending correction folded into the node voltages
-// (X <- X + dx, dx <- 0: the same sum the next replay would form).  The
-// graphs are captured again for the smaller batch.  Every kernel's
+// into a denser working batch in one of two alternate sets (in the
+// correction form the node voltages X[parity] hold every instance's current
+// iterate, or its final voltages once frozen).  The graphs are captured
+// again for the smaller batch.  Every kernel's
 // arithmetic per instance is independent of the batch width and of the
 // other instances, so results are bit-identical with or without compaction.
 
@@ -1298,21 +1336,20 @@ int alloc_alt(pvglu_pv *pv) {
 
 template <typename T>
 void gather(hipStream_t s, long rows, int Bc, int Bn, const int *sel,
-            const T *in, const T *add, T *out) {
+            const T *in, T *out) {
   const long cnt = rows * Bn;
   if (cnt > 0)
     hipLaunchKernelGGL(k_gather<T>, dim3((unsigned)((cnt + 255) / 256)),
-                       dim3(256), 0, s, rows, Bc, Bn, sel, in, add, out);
+                       dim3(256), 0, s, rows, Bc, Bn, sel, in, out);
 }
 
 template <typename T>
 void scatter(hipStream_t s, long rows, int Bc, int Bh, int cnt, const int *from,
-             const int *to, const T *in, const T *add, T *out) {
+             const int *to, const T *in, T *out) {
   const long tot = rows * cnt;
   if (tot > 0)
     hipLaunchKernelGGL(k_scatter<T>, dim3((unsigned)((tot + 255) / 256)),
-                       dim3(256), 0, s, rows, Bc, Bh, cnt, from, to, in, add,
-                       out);
+                       dim3(256), 0, s, rows, Bc, Bh, cnt, from, to, in, out);
 }
 
 struct Compaction {
@@ -1344,15 +1381,11 @@ struct Compaction {
                     bool corr) {
     NewtonDev *d = pv->nd;
     const InstSet w = work_set(d);
-    scatter<double>(d->s, pv->nmod, Bc, B, cnt, from, to, w.v, nullptr, home.v);
-    if (corr)
-      scatter<double>(d->s, pv->n, Bc, B, cnt, from, to, parity ? w.X1 : w.X0,
-                      w.dx, home.x);
-    else
-      scatter<double>(d->s, pv->n, Bc, B, cnt, from, to, w.x, nullptr, home.x);
-    scatter<int>(d->s, 1, Bc, B, cnt, from, to, w.iters, nullptr, home.iters);
-    scatter<double>(d->s, 1, Bc, B, cnt, from, to, w.lastmax, nullptr,
-                    home.lastmax);
+    scatter<double>(d->s, pv->nmod, Bc, B, cnt, from, to, w.v, home.v);
+    scatter<double>(d->s, pv->n, Bc, B, cnt, from, to,
+                    corr ? (parity ? w.X1 : w.X0) : w.x, home.x);
+    scatter<int>(d->s, 1, Bc, B, cnt, from, to, w.iters, home.iters);
+    scatter<double>(d->s, 1, Bc, B, cnt, from, to, w.lastmax, home.lastmax);
   }
 
   int compact(int parity, bool corr) {
@@ -1389,23 +1422,23 @@ struct Compaction {
     const InstSet w = work_set(d);
     InstSet nx = d->alt[next];
     const long nm = pv->nmod, n = pv->n;
-    gather<double>(d->s, nm, Bc, Bn, sel, w.G, nullptr, nx.G);
-    gather<double>(d->s, nm, Bc, Bn, sel, w.v, nullptr, nx.v);
-    gather<double>(d->s, nm, Bc, Bn, sel, w.geq, nullptr, nx.geq);
-    gather<double>(d->s, nm, Bc, Bn, sel, w.ieq, nullptr, nx.ieq);
-    if (corr) {
-      gather<double>(d->s, n, Bc, Bn, sel, parity ? w.X1 : w.X0, w.dx,
+    gather<double>(d->s, nm, Bc, Bn, sel, w.G, nx.G);
+    gather<double>(d->s, nm, Bc, Bn, sel, w.v, nx.v);
+    gather<double>(d->s, nm, Bc, Bn, sel, w.geq, nx.geq);
+    gather<double>(d->s, nm, Bc, Bn, sel, w.ieq, nx.ieq);
+    if (corr) // the correction is formed and used within a replay
+      gather<double>(d->s, n, Bc, Bn, sel, parity ? w.X1 : w.X0,
                      parity ? nx.X1 : nx.X0);
-      hipLaunchKernelGGL(k_zero, grid_for(n * Bn), dim3(256), 0, d->s,
-                         n * Bn, nx.dx);
-    } // direct form: the next replay's solve writes every x
-    gather<double>(d->s, 1, Bc, Bn, sel, w.gvs, nullptr, nx.gvs);
-    gather<double>(d->s, 1, Bc, Bn, sel, w.lastmax, nullptr, nx.lastmax);
-    gather<int>(d->s, 1, Bc, Bn, sel, w.stage, nullptr, nx.stage);
-    gather<int>(d->s, 1, Bc, Bn, sel, w.stall, nullptr, nx.stall);
-    gather<int>(d->s, 1, Bc, Bn, sel, w.iters, nullptr, nx.iters);
+    // direct form: the next replay's solve writes every x
+    gather<double>(d->s, 1, Bc, Bn, sel, w.gvs, nx.gvs);
+    gather<double>(d->s, 1, Bc, Bn, sel, w.lastmax, nx.lastmax);
+    gather<int>(d->s, 1, Bc, Bn, sel, w.stage, nx.stage);
+    gather<int>(d->s, 1, Bc, Bn, sel, w.stall, nx.stall);
+    gather<int>(d->s, 1, Bc, Bn, sel, w.iters, nx.iters);
     hipLaunchKernelGGL(k_fill_i32, grid_for(Bn), dim3(256), 0, d->s, Bn,
                        d->active, 1);
+    hipLaunchKernelGGL(k_fill_i32, grid_for(Bn), dim3(256), 0, d->s, Bn,
+                       d->justf, 0);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(d->s));
     // the home set's x keeps the caller's results; the working set brings
@@ -1478,13 +1511,12 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
   const bool corr = d->form == PVGLU_NEWTON_CORRECTION;
   hipLaunchKernelGGL(k_newton_init, grid_for(B), dim3(256), 0, d->s, B,
                      kLadder[12 - nst], d->stage, d->gvs, d->active, d->stall,
-                     d->iters, d->lastmax, d->dsum, d->dmax);
+                     d->iters, d->lastmax, d->dsum, d->dmax, d->justf);
   if (corr) {
-    // node voltages start at 0 with a zero correction: the first
-    // iteration's right-hand side is the companion b (its x = A^-1 b)
+    // node voltages start at 0: the first iteration's right-hand side is
+    // the companion b (its correction x = A^-1 b)
     const long cnt = (long)pv->n * B;
     hipLaunchKernelGGL(k_zero, grid_for(cnt), dim3(256), 0, d->s, cnt, d->X[0]);
-    hipLaunchKernelGGL(k_zero, grid_for(cnt), dim3(256), 0, d->s, cnt, d->dx);
   }
   launch_eval(pv, B, d->active); // the first replay's companion models
   HIP_TRY(hipGetLastError());
@@ -1494,10 +1526,11 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
   // replays: a replay after an instance has converged leaves it unchanged
   // (frozen: no evaluation, stamping or update; in the direct form the
   // refactor + solve of its unchanged system reproduces its solution bit for
-  // bit, in the correction form its right-hand side and so its correction
-  // are zero), so the extra replays of a burst cost time only, and the GPU
-  // no longer idles for a host round trip after each iteration.  The
-  // correction form alternates its two graphs (replay j reads X[j % 2]).
+  // bit, in the correction form its node voltages stay in both buffers and
+  // its right-hand side is zero), so the extra replays of a burst cost time
+  // only, and the GPU no longer idles for a host round trip after each
+  // iteration.  The correction form alternates its two graphs (replay j
+  // reads X[j % 2]).
   // With skip_frozen, a poll that finds at most three quarters of the
   // working batch active compacts it (Compaction above).
   constexpr int kPoll = 4;
@@ -1532,12 +1565,11 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
     if ((st = cm.finish(parity, corr)))
       return st;
   } else if (corr) {
-    // the last replay wrote X[parity]; its correction is still in dx (zero
-    // for the instances frozen before it)
-    const long cnt = (long)pv->n * B;
-    hipLaunchKernelGGL(k_add, grid_for((cnt + 3) / 4), dim3(256), 0, d->s, cnt,
-                       d->X[parity], d->dx, d->x, B, nullptr);
-    HIP_TRY(hipGetLastError());
+    // X[parity] holds every instance's final node voltages: the update of
+    // the replay that froze it wrote them there, and the update of the next
+    // replay, if any, copied them to the other buffer
+    HIP_TRY(hipMemcpyAsync(d->x, d->X[parity], sizeof(double) * pv->n * B,
+                           hipMemcpyDeviceToDevice, d->s));
     HIP_TRY(hipStreamSynchronize(d->s));
   }
   auto t1 = std::chrono::steady_clock::now();
