@@ -182,6 +182,11 @@ __global__ void k_eval(int nm, int B, const double *__restrict__ Girr,
   ieq[gid] = f - g * V;
 }
 
+// The module operands of an element's evaluation, loaded with its update.
+struct ModuleOperands {
+  double g, rsh, lk;
+};
+
 // 011 update fused with the companion models of the next iteration: once a
 // module's voltage is updated (update_elements), its conductance and current
 // source for the next stamping are evaluated from the new voltage in
@@ -190,8 +195,8 @@ __global__ void k_eval(int nm, int B, const double *__restrict__ Girr,
 // control kernel freezes right after this update gets an evaluation that
 // nothing reads (stamping skips frozen instances); the control kernel does
 // not change voltages.
-// Direct form (dx null): x holds the new node voltages; ieq <- f - G v.
-// Correction form: x + dx are the new node voltages (x = the previous ones,
+// Direct form (CORR false, dx null): x holds the new node voltages;
+// ieq <- f - G v.  Correction form (CORR): x + dx are the new node voltages (x = the previous ones,
 // dx = the solve's correction) and ieq <- f + G (V - v), the linearized
 // branch current at those node voltages, which the next iteration's residual
 // gathers (V - v = 0 exactly when the step was not damped).  The update also
@@ -200,11 +205,6 @@ __global__ void k_eval(int nm, int B, const double *__restrict__ Girr,
 // the blocks past the nblk element blocks, at the nodes left[0..nleft) that
 // no element reaches.  The additions are those the next iteration would
 // otherwise make over the whole vector, so the iterates are the same bits.
-// The module operands of an element's evaluation, loaded with its update.
-struct ModuleOperands {
-  double g, rsh, lk;
-};
-
 template <bool CORR>
 __global__ void __launch_bounds__(256)
     k_update_eval(int ne, int B, const int *__restrict__ pos,
@@ -223,7 +223,8 @@ __global__ void __launch_bounds__(256)
     *remaining = 0;
   if (CORR && (int)blockIdx.x >= nblk) { // nodes without an element terminal
     const int lt = upd_tile_log2(B), rp = 256 >> lt;
-    const int in = (int)(blockIdx.y << lt) + (int)(threadIdx.x & ((1u << lt) - 1u));
+    const int in =
+        (int)(blockIdx.y << lt) + (int)(threadIdx.x & ((1u << lt) - 1u));
     if (in >= B)
       return;
     const bool live = active[in] != 0, copy = !live && justf[in] != 0;
@@ -265,9 +266,8 @@ __global__ void __launch_bounds__(256)
 // string's ~600 V node voltages cancel exactly and the residual carries the
 // rounding of the branch currents only.  Frozen instances get a zero
 // right-hand side (their correction is not used: the update skips them).
-// Layout: one block
-// per kStampRows rows x up to 256 instances, as the stamping gathers; a
-// string row reads its two modules' branch currents only.  `rows` (may be
+// Layout: one block per kStampRows rows x up to 256 instances, as the
+// stamping gathers; a string row reads its two modules' branch currents only.  `rows` (may be
 // null: rows 0 .. n-1) lists the rows to compute: with the right-hand side
 // formed on load by the stamped refactor, only the rows it does not form.
 __global__ void __launch_bounds__(256)
