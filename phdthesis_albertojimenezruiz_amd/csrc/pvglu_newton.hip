@@ -115,6 +115,10 @@ struct NewtonDev {
   bool res_on_load = false;
   int fused = 1;        // stamping on load requested when captured
   hipStream_t s = nullptr;
+  // the pipelined poll of the batched loop: the remaining-instance count of
+  // each burst lands in pinned host memory, behind an event
+  int *hrem = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
   // one graph per iteration; the correction form alternates two (parity p
   // reads node voltages X[p] and its update writes X[1 - p])
   hipGraphExec_t exec = nullptr, exec2 = nullptr;
@@ -147,6 +151,11 @@ void newton_dev_destroy(NewtonDev *d) {
       (void)hipGraphDestroy(g);
   if (d->s)
     (void)hipStreamDestroy(d->s);
+  for (hipEvent_t e : d->ev)
+    if (e)
+      (void)hipEventDestroy(e);
+  if (d->hrem)
+    (void)hipHostFree(d->hrem);
   for (void *p : d->allocs)
     (void)hipFree(p);
   for (void *p : d->inst_allocs)
@@ -992,6 +1001,9 @@ int newton_setup(pvglu_pv *pv) {
   d->h_stamp_seg.swap(sseg);
   d->nleft = (int)left.size();
   HIP_TRY(hipStreamCreate(&d->s));
+  for (hipEvent_t &e : d->ev)
+    HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  HIP_TRY(hipHostMalloc((void **)&d->hrem, 2 * sizeof(int)));
   if (pv->nmod > 0) {
     hipLaunchKernelGGL(k_module_lk, dim3((pv->nmod + 255) / 256), dim3(256), 0,
                        d->s, pv->nmod, d->rsh, d->lk);
@@ -1529,15 +1541,59 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
   // bit, in the correction form its node voltages stay in both buffers and
   // its right-hand side is zero), so the extra replays of a burst cost time
   // only, and the GPU no longer idles for a host round trip after each
-  // iteration.  The correction form alternates its two graphs (replay j
-  // reads X[j % 2]).
-  // With skip_frozen, a poll that finds at most three quarters of the
-  // working batch active compacts it (Compaction above).
+  // iteration.  The poll is pipelined: the next burst is queued before the
+  // host waits for the previous one's count, so the GPU does not idle for
+  // the round trip of a poll either (a burst queued behind the one that
+  // finished every instance changes nothing; its replays are not counted).
+  // The correction form alternates its two graphs (replay j reads
+  // X[j % 2]).  With skip_frozen, a poll that finds at most three quarters
+  // of the working batch active compacts it (Compaction above), after the
+  // burst behind it.
   constexpr int kPoll = 4;
   int iters = 0, remaining = 1, parity = 0;
   Compaction cm(pv, B);
+  auto launch_burst = [&](int slot) -> int {
+    const int burst = std::min(kPoll, max_iters - iters);
+    for (int k = 0; k < burst; ++k) {
+      HIP_TRY(hipGraphLaunch(parity ? d->exec2 : d->exec, d->s));
+      if (corr)
+        parity ^= 1;
+    }
+    iters += burst;
+    HIP_TRY(hipMemcpyAsync(&d->hrem[slot], d->remaining, sizeof(int),
+                           hipMemcpyDeviceToHost, d->s));
+    HIP_TRY(hipEventRecord(d->ev[slot], d->s));
+    return PVGLU_OK;
+  };
   auto t0 = std::chrono::steady_clock::now();
-  while (remaining > 0) {
+  int slot = 0;
+  if ((st = launch_burst(slot)))
+    return st;
+  for (;;) {
+    int ahead = 0; // replays queued behind the polled burst
+    if (iters < max_iters) {
+      const int before = iters;
+      if ((st = launch_burst(slot ^ 1)))
+        return st;
+      ahead = iters - before;
+    }
+    HIP_TRY(hipEventSynchronize(d->ev[slot]));
+    remaining = d->hrem[slot];
+    const bool compact =
+        pv->skip_frozen && remaining > 0 && 4 * remaining <= 3 * cm.Bc;
+    if (remaining > 0 && ahead > 0 && !compact) {
+      slot ^= 1; // poll the burst behind next
+      continue;
+    }
+    HIP_TRY(hipStreamSynchronize(d->s));
+    if (ahead > 0) {
+      if (remaining == 0)
+        iters -= ahead; // replays of finished instances only
+      else
+        remaining = d->hrem[slot ^ 1];
+    }
+    if (remaining == 0)
+      break;
     if (iters >= max_iters) {
       // hand back the current iterate for diagnosis
       if (cm.active && (st = cm.finish(parity, corr)))
@@ -1547,19 +1603,12 @@ static int batched_solve(pvglu_pv *pv, int B, const double *g, double *v_module,
       return fail(PVGLU_E_NOCONV, "batched Newton: %d iterations, %d instances left",
                   iters, remaining);
     }
-    const int burst = std::min(kPoll, max_iters - iters);
-    for (int k = 0; k < burst; ++k) {
-      HIP_TRY(hipGraphLaunch(parity ? d->exec2 : d->exec, d->s));
-      if (corr)
-        parity ^= 1;
-    }
-    iters += burst;
-    HIP_TRY(hipMemcpyAsync(&remaining, d->remaining, sizeof(int),
-                           hipMemcpyDeviceToHost, d->s));
-    HIP_TRY(hipStreamSynchronize(d->s));
-    if (pv->skip_frozen && remaining > 0 && 4 * remaining <= 3 * cm.Bc)
+    if (pv->skip_frozen && 4 * remaining <= 3 * cm.Bc)
       if ((st = cm.compact(parity, corr)))
         return st;
+    slot = 0;
+    if ((st = launch_burst(slot)))
+      return st;
   }
   if (cm.active) {
     if ((st = cm.finish(parity, corr)))
