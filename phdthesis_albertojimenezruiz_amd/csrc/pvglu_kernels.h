@@ -608,13 +608,13 @@ __device__ __forceinline__ void tri_lu_segment_stamped(
   const double r0s = (rb & 4) ? -1.0 : 1.0, r1s = (rb & 8) ? -1.0 : 1.0;
   double xprv = 0.0, xcur = 0.0; // node voltages of the previous / this row
   if constexpr (RES && XH) {
-    xprv = xk[(size_t)(col0 - dcol) * B + inst];
-    xcur = xk[(size_t)col0 * B + inst];
+    xprv = ldv<STREAM>(&xk[(size_t)(col0 - dcol) * B + inst]);
+    xcur = ldv<STREAM>(&xk[(size_t)col0 * B + inst]);
   }
   double gup = 0.0, iup = 0.0; // chained: the U-above module's g and i
   if constexpr (RES) {
-    gup = geq[(size_t)mu0 * B + inst];
-    iup = ieq[(size_t)mu0 * B + inst];
+    gup = ldv<STREAM>(&geq[(size_t)mu0 * B + inst]);
+    iup = ldv<STREAM>(&ieq[(size_t)mu0 * B + inst]);
   }
   double pl = sel4(p0, p1, p2, p3, k);
   double v0 = 0.0, v1 = 0.0, v2 = 0.0, minpiv = 1.0;
@@ -628,15 +628,21 @@ __device__ __forceinline__ void tri_lu_segment_stamped(
       len,
       [&](ColVals &c, int i) {
         i = min(i, len - 1);
-        // cached loads: a string's next column reads one of the two again
+        // without RES a string's next column reads one of the two module
+        // values again (cached loads); chained (RES), each is read once and
+        // not again before the update overwrites it (streaming loads)
         if constexpr (!RES)
           c.a0 = geq[(size_t)(mu0 + i * dmu) * B + inst];
-        c.a2 = geq[(size_t)(ml0 + i * dml) * B + inst];
+        if constexpr (RES)
+          c.a2 = ldv<STREAM>(&geq[(size_t)(ml0 + i * dml) * B + inst]);
+        else
+          c.a2 = geq[(size_t)(ml0 + i * dml) * B + inst];
         if constexpr (RES) { // the L-below module's branch current
-          c.a3 = ieq[(size_t)(ml0 + i * dml) * B + inst];
+          c.a3 = ldv<STREAM>(&ieq[(size_t)(ml0 + i * dml) * B + inst]);
           if constexpr (XH) // the next node (of the last column: sr[7])
-            c.xn = xk[(size_t)(i + 1 < len ? col0 + (i + 1) * dcol : sr[7]) *
-                          B + inst];
+            c.xn = ldv<STREAM>(
+                &xk[(size_t)(i + 1 < len ? col0 + (i + 1) * dcol : sr[7]) * B +
+                    inst]);
         } else {
           c.bb = ldv<STREAM>(&b[(size_t)(col0 + i * dcol) * B + inst]);
         }
