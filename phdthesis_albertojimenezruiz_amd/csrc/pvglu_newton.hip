@@ -257,8 +257,11 @@ __global__ void __launch_bounds__(256)
         const ModuleEval me =
             module_current_lk(V, kIphPerIrradiance * m.g, m.rsh, m.lk);
         const double f = -me.I, g = -me.dIdV;
-        geq[gid] = g;
-        ieq[gid] = CORR ? f + g * (Vx - V) : f - g * V;
+        // read once, by the next refactor's streaming loads (or the
+        // stamping kernels), after a whole sweep of other traffic
+        __builtin_nontemporal_store(g, &geq[gid]);
+        __builtin_nontemporal_store(CORR ? f + g * (Vx - V) : f - g * V,
+                                    &ieq[gid]);
       },
       xo, justf);
 }
